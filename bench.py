@@ -1,0 +1,204 @@
+"""Benchmark of the TICA-moments + assignment path on one MI355X.
+
+Workload (default): 200 trajectories x 50,000 frames x 300 float32 features
+(pairwise distances of 25 Brownian particles in a 3-basin bias, generated on
+the GPU), TICA lag 20 and assignment to k = 1000 centers (kmeans++ on a
+200k-frame subsample, outside the timed region).
+
+One step = lagged moments of all pairs (bf16 hi/lo split on MFMA, FP64 merge)
+-> host FP64 eigensolve, concurrently with the assignment of every frame
+(bf16x3 MFMA + exact recheck) on a second stream.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s TICA+assign (d=300,k=1000) vs plain GPU & CPU; serving p99 ms"
+PEAK_BF16_TFLOPS = 2516.6
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ntraj", type=int, default=200)
+    ap.add_argument("--nframes", type=int, default=50000)
+    ap.add_argument("--k", type=int, default=1000)
+    ap.add_argument("--lag", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 GPU comparison")
+    ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
+    ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from pyemma_amd import _native, _solvers
+    from pyemma_amd.synthetic import brownian_distances
+
+    dev = _native.device()
+    t0 = time.time()
+    X, offsets = brownian_distances(args.ntraj, args.nframes, seed=2, device=dev)
+    torch.cuda.synchronize()
+    N, d = X.shape
+    log("data %d x %d generated in %.1f s" % (N, d, time.time() - t0))
+    off_dev = torch.as_tensor(offsets, device=dev)
+    sub = X[::max(1, N // 200000)].contiguous()
+    C, _ = _native.kmeanspp(sub, args.k, 12345)
+    torch.cuda.synchronize()
+    log("centers: kmeans++ k=%d on %d frames" % (args.k, sub.shape[0]))
+
+    s_mom = torch.cuda.current_stream()
+    s_asg = torch.cuda.Stream()
+    labels = torch.empty(N, dtype=torch.int32, device=dev)
+    nre = torch.zeros(1, dtype=torch.int64, device=dev)
+    ev = {k: torch.cuda.Event(enable_timing=True) for k in ("m0", "m1")}
+    ea = {k: torch.cuda.Event(enable_timing=True) for k in ("a0", "a1")}
+
+    def step(prec=_native.EMMA_PREC_BF16X4, amode=_native.EMMA_ASSIGN_BF16X3_RECHECK):
+        s_asg.wait_stream(s_mom)
+        ev["m0"].record(s_mom)
+        w, sx, sy, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True,
+                                                        c0t=True, precision=prec)
+        ev["m1"].record(s_mom)
+        with torch.cuda.stream(s_asg):
+            ea["a0"].record(s_asg)
+            _native.assign(X, C, mode=amode, labels=labels, n_rechecked=nre)
+            ea["a1"].record(s_asg)
+        wh = w.item()  # waits for the moments only
+        C00 = Mxx.cpu().numpy() / wh
+        C0t = Mxy.cpu().numpy() / wh
+        l, R = _solvers.eig_corr(C00, C0t, 1e-6, sign_maxelement=True)
+        torch.cuda.current_stream().wait_stream(s_asg)
+        torch.cuda.synchronize()
+        return l, ev["m0"].elapsed_time(ev["m1"]), ea["a0"].elapsed_time(ea["a1"])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times, tm, ta = [], [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        l, mms, ams = step()
+        times.append(time.perf_counter() - t1)
+        tm.append(mms)
+        ta.append(ams)
+    torch.cuda.synchronize()
+    step_s = float(np.mean(times))
+    value = N / step_s
+    n_re = int(nre.item())
+    P = int(sum(max(0, (b - a) - args.lag) for a, b in zip(offsets[:-1], offsets[1:])))
+    mom_ms, asg_ms = float(np.mean(tm)), float(np.mean(ta))
+    f_mom = 6.0 * d * d * P
+    f_asg = 2.0 * d * args.k * N
+    kern = {
+        "moments": {"ms": mom_ms, "algorithmic_tflop": f_mom / 1e12, "tflops": f_mom / (mom_ms * 1e-3) / 1e12},
+        "assign": {"ms": asg_ms, "algorithmic_tflop": f_asg / 1e12, "tflops": f_asg / (asg_ms * 1e-3) / 1e12,
+                   "rechecked_frames": n_re},
+    }
+    dom = "assign" if asg_ms >= mom_ms else "moments"
+    ach = kern[dom]["tflops"]
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_BF16_TFLOPS, "traffic": None}
+    log("step %.2f ms (moments %.2f ms, assign %.2f ms, rechecked %d)" % (step_s * 1e3, mom_ms, asg_ms, n_re))
+
+    plain = None
+    if not args.no_plain:
+        t1 = time.perf_counter()
+        step(_native.EMMA_PREC_FP32, _native.EMMA_ASSIGN_FP32_PLAIN)
+        tp = time.perf_counter() - t1
+        plain = {"value": N / tp, "unit": "frames/s", "ms_per_step": tp * 1e3,
+                 "moments_ms": ev["m0"].elapsed_time(ev["m1"]), "assign_ms": ea["a0"].elapsed_time(ea["a1"]),
+                 "kind": "FP32 MFMA moments + sequential FP32 assign loop"}
+        log("plain GPU step %.1f ms" % (tp * 1e3))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(args, X, offsets, C, labels, l)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16x4 moments / bf16x3+fp32-recheck assign, fp64 merge",
+        "data": "synthetic",
+        "config": {"workload": "%d traj x %d frames x %d f32 (Brownian pairwise distances), TICA lag %d, "
+                               "assign k=%d" % (args.ntraj, args.nframes, d, args.lag, args.k),
+                   "frames": N, "lagged_pairs": P},
+        "roofline": roofline, "kernels": kern, "plain_gpu": plain, "cpu_baseline": cpu, "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, X, offsets, C, labels_gpu, l_gpu):
+    """The oracle (CPU restatement of the reference path) on a bounded sample."""
+    import torch
+
+    from oracle import eig as oeig
+    from oracle import moments as om
+    from oracle import native as on
+    from pyemma_amd import _native
+
+    cores = len(os.sched_getaffinity(0))
+    N, d = X.shape
+    # moments sample: whole trajectories
+    nt = max(1, args.cpu_frames // args.nframes)
+    sample = [X[offsets[i]:offsets[i + 1]].cpu().numpy() for i in range(nt)]
+    t0 = time.perf_counter()
+    ref = om.covariance_lagged(sample, args.lag, remove_data_mean=True, reversible=True, bessel=False)
+    t_mom = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    from pyemma_amd import _solvers
+    _solvers.eig_corr(ref["C00"], ref["C0t"], 1e-6, sign_maxelement=True)
+    t_eig = time.perf_counter() - t0
+    na = args.cpu_assign_frames
+    Xa = X[:na].cpu().numpy()
+    Ch = C.cpu().numpy()
+    t0 = time.perf_counter()
+    lab_ref = on.assign(Xa, Ch, nthreads=cores)
+    t_asg = time.perf_counter() - t0
+    n_m = sum(len(s) for s in sample)
+    t_full = t_mom * N / n_m + t_eig + t_asg * N / na
+    cpu = {"value": N / t_full, "unit": "frames/s", "cores": cores, "kind": "port",
+           "sample": "moments: %d traj (%d frames, numpy FP64 dgemm restatement, %.2f s); eig d=%d (%.3f s); "
+                     "assign: %d frames x k=%d (C oracle, OpenMP %d threads, %.2f s); scaled to %d frames"
+                     % (nt, n_m, t_mom, d, t_eig, na, Ch.shape[0], cores, t_asg, N)}
+    # parity on the same sample: labels and eigenvalues of the GPU path
+    lab_gpu = labels_gpu[:na].cpu().numpy()
+    off_s = np.concatenate([[0], np.cumsum([len(s) for s in sample])]).astype(np.int64)
+    Xs = torch.as_tensor(np.concatenate(sample), device=X.device)
+    w, sx, sy, Mxx, Mxy, _ = _native.moments_lagged(Xs, off_s, args.lag, reversible=True, remove_mean=True,
+                                                    c0t=True, precision=_native.EMMA_PREC_BF16X4)
+    wh = w.item()
+    ev_gpu = oeig.generalized_eigenvalues(Mxx.cpu().numpy() / wh, Mxy.cpu().numpy() / wh)
+    ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
+    rel = np.abs(ev_gpu - ev_ref) / np.abs(ev_ref)
+    parity = {"labels_identical": bool(np.array_equal(lab_gpu, lab_ref)), "label_frames": na,
+              "eig_rel_err_max": float(rel.max()), "eig_rel_err_top10_max": float(rel[:10].max()),
+              "eig_frames": n_m}
+    log("cpu baseline %.3g frames/s on %d cores; parity %s" % (cpu["value"], cores, parity))
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * emma_amd.h — C ABI of the MI355X discretization library (libemma_amd.so).
+ *
+ * The library computes, on one AMD Instinct MI355X (gfx950):
+ *   - TICA's lagged mean-free moments (mean, C00, C0t, optionally Ctt),
+ *   - assignment of frames to cluster centers (argmin of squared distance),
+ *   - kmeans++ seeding and Lloyd iterations,
+ *   - the TICA projection (X - mean) V.
+ *
+ * Conventions
+ *   - Every pointer argument is a DEVICE pointer owned by the caller unless
+ *     the comment says otherwise.  Frame matrices are row-major, C-contiguous
+ *     float32 (the layout of np.require(X, float32, 'C'),
+ *     clustering/interface.py:159).  Labels are int32, moments FP64.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Every call only
+ *     enqueues work on that stream; nothing synchronizes the host.
+ *   - Scratch memory is passed in as `workspace`; its size comes from the
+ *     matching *_workspace_size() call.  The library itself allocates no
+ *     device memory.
+ *   - Return value: EMMA_OK (0) or a negative EMMA_ERR_* code; the message of
+ *     the last failure on the calling thread is emma_last_error().
+ *   - Results are deterministic: the same inputs give bit-identical outputs
+ *     on every run.
+ *
+ * Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the PyEMMA source tree).
+ */
+#ifndef EMMA_AMD_H
+#define EMMA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define EMMA_OK 0
+#define EMMA_ERR_INVALID_ARG (-1)
+#define EMMA_ERR_HIP (-2)
+#define EMMA_ERR_WORKSPACE (-3)
+#define EMMA_ERR_UNSUPPORTED (-4)
+
+/* moment flags */
+#define EMMA_MOM_SYMMETRIZE 1u   /* reversible: sx=sy=sum X+sum Y, Cxx=X'X+Y'Y, Cxy=X'Y+Y'X */
+#define EMMA_MOM_REMOVE_MEAN 2u  /* mean-free (co)moments */
+#define EMMA_MOM_C0T 4u          /* compute Mxy */
+#define EMMA_MOM_CTT 8u          /* compute Myy = Y'Y (not with SYMMETRIZE) */
+
+/* arithmetic of the covariance tiles */
+#define EMMA_PREC_FP32 0    /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 tile accumulation */
+#define EMMA_PREC_BF16X4 1  /* operands split into bf16 hi+lo, 4 bf16 MFMA products, FP32 tile accumulation */
+
+/* assignment modes */
+#define EMMA_ASSIGN_FP32_PLAIN 0    /* sequential FP32 loop per (frame, center) on the vector ALUs */
+#define EMMA_ASSIGN_BF16X3_RECHECK 1 /* bf16x3 MFMA distances + near-tie recheck with the FP32 loop */
+
+/* Library version string (host pointer, static storage). */
+const char *emma_version(void);
+
+/* Message of the last failed call on this thread (host pointer). */
+const char *emma_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Lagged moments                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Workspace bytes needed by emma_moments_lagged for these arguments. */
+size_t emma_moments_workspace_size(int64_t n_rows, int32_t d, int32_t ntraj, uint32_t flags,
+                                   int32_t precision);
+
+/*
+ * Unnormalized lagged moments of many trajectories in one call.
+ *
+ * Replaces the chunk loop of LaggedCovariance._estimate
+ * (coordinates/estimation/covariance.py:219-244) over RunningCovar.add
+ * (_ext/variational/estimators/running_moments.py:233-308) with
+ * moments_XXXY / moments_block / moments_XX (moments.py:601-973) and the
+ * Chan merge Moments.combine (running_moments.py:35-61).
+ *
+ * X            frames of all trajectories, rows concatenated: (n_rows, d) float32
+ * traj_offsets device int64[ntraj+1], trajectory t is rows [off[t], off[t+1])
+ * lag, stride, skip  pairing of iterable.py:95-160: pair i of a trajectory is
+ *              (row skip+i*stride, row skip+lag+i*stride); lag == 0 means
+ *              instantaneous moments of rows skip, skip+stride, ... (C00 only)
+ * flags        EMMA_MOM_*
+ * precision    EMMA_PREC_*
+ * outputs (FP64, device): w[1]; sx[d]; sy[d] (may be NULL); Mxx[d*d];
+ *              Mxy[d*d] (needs EMMA_MOM_C0T); Myy[d*d] (needs EMMA_MOM_CTT).
+ *              The values follow the reference: with SYMMETRIZE w = 2P, sx = sy
+ *              = sum of X and Y rows; otherwise w = P, sx = sum X, sy = sum Y.
+ *              With REMOVE_MEAN the matrices are centered at sx/w (and sy/w).
+ * P == 0 (no trajectory longer than skip+lag) writes w = 0 and zeros.
+ */
+int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, const int64_t *traj_offsets,
+                        int32_t ntraj, int32_t lag, int32_t stride, int32_t skip, uint32_t flags,
+                        int32_t precision, double *w, double *sx, double *sy, double *Mxx,
+                        double *Mxy, double *Myy, void *workspace, size_t workspace_bytes,
+                        void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Assignment                                                          */
+/* ------------------------------------------------------------------ */
+
+/* Workspace bytes needed by emma_assign for these arguments. */
+size_t emma_assign_workspace_size(int64_t n, int32_t d, int32_t k, int32_t mode);
+
+/*
+ * labels[i] = argmin_j sum_c (X[i,c] - C[j,c])^2, the sum taken in FP32 in
+ * index order without fused multiply-add, first minimum wins.
+ *
+ * Replaces ClusterModel(cluster_centers, metric='euclidean').transform(X)
+ * as called from AbstractClustering._transform_array
+ * (coordinates/clustering/interface.py:157-167; API assign_to_centers,
+ * coordinates/api.py:2060-2156).
+ *
+ * mode EMMA_ASSIGN_BF16X3_RECHECK computes candidate distances with bf16 hi/lo
+ * MFMA products and re-evaluates, with the FP32 loop, every frame whose two
+ * best candidates lie within a rigorous error bound; labels are identical to
+ * EMMA_ASSIGN_FP32_PLAIN.  n_rechecked (device int64, may be NULL) receives the
+ * number of re-evaluated frames.
+ */
+int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k, int32_t *labels,
+                int32_t mode, int64_t *n_rechecked, void *workspace, size_t workspace_bytes,
+                void *stream);
+
+/* ------------------------------------------------------------------ */
+/* k-means                                                             */
+/* ------------------------------------------------------------------ */
+
+size_t emma_kmeanspp_workspace_size(int64_t n, int32_t d, int32_t k);
+
+/*
+ * kmeans++ seeding: C_out (k,d) float32 rows of X, idx_out (k,) int64 row
+ * indices.  Round 0 takes row floor(u_0 n); round r takes a row with
+ * probability proportional to its squared FP32 distance to the nearest
+ * chosen center, u_r = SplitMix64 draw r of `seed`, with the FP64 summation
+ * order documented in oracle/emma_oracle.c (oracle_kmpp_pick).
+ *
+ * Replaces deeptime's kmeans++ initialization invoked by
+ * KmeansClustering._estimate (coordinates/clustering/kmeans.py:211-212,
+ * 254-258; entry observed as init_centers_kmpp, clustering/tests/
+ * test_kmeans.py:304).
+ */
+int emma_kmeanspp(const float *X, int64_t n, int32_t d, int32_t k, uint64_t seed, float *C_out,
+                  int64_t *idx_out, void *workspace, size_t workspace_bytes, void *stream);
+
+size_t emma_lloyd_workspace_size(int64_t n, int32_t d, int32_t k, int32_t assign_mode);
+
+/*
+ * Lloyd iterations on centers C (k,d) float32, updated in place.  Each
+ * iteration: labels = emma_assign(X, C); cost = blocked FP64 sum of every
+ * frame's FP32 distance to its center; C_j = mean of its members (FP64,
+ * documented order; empty centers unchanged); stop when
+ * |cost - prev| / cost <= tol (prev starts at 0).
+ * Device outputs: labels (n,) of the last assignment, inertias[max_iter]
+ * (cost per iteration run), n_iter[1] (int32), converged[1] (int32).
+ *
+ * Replaces deeptime KMeans.fit's Lloyd loop as called at
+ * coordinates/clustering/kmeans.py:254-261 (tolerance rule kmeans.py:67-75).
+ */
+int emma_lloyd(const float *X, int64_t n, int32_t d, float *C, int32_t k, int32_t max_iter,
+               double tol, int32_t assign_mode, int32_t *labels, double *inertias,
+               int32_t *n_iter, int32_t *converged, void *workspace, size_t workspace_bytes,
+               void *stream);
+
+/* ------------------------------------------------------------------ */
+/* TICA projection and input checks                                    */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Y = (X - mean) V[:, :m], computed in FP64 and rounded to float32.
+ * mean (d,) FP64, V (d, ldv) FP64 row-major (columns 0..m-1 used), Y (n, m).
+ *
+ * Replaces TICABase._transform_array
+ * (coordinates/transform/_tica_base.py:117-133).
+ */
+int emma_project(const float *X, int64_t n, int32_t d, const double *mean, const double *V,
+                 int32_t ldv, int32_t m, float *Y, void *stream);
+
+/*
+ * count[0] = number of non-finite entries of X (n elements, float32).
+ * Replaces the per-chunk np.isfinite check of DataSourceIterator.__next__
+ * (coordinates/data/_base/datasource.py:1067-1075).  count is device int64.
+ */
+int emma_count_nonfinite(const float *X, int64_t n, int64_t *count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMMA_AMD_H */

@@ -1,0 +1,85 @@
+"""Host FP64 solvers of the TICA eigenproblem (d x d, LAPACK through scipy).
+
+Restates pyemma/_ext/variational/solvers/direct.py:7-295.  The reference's
+eig_qr (solvers/eig_qr/eig_qr.pyx:4-61) runs LAPACK dsytrd -> dorgtr ->
+dsteqr on the upper triangle; that is exactly the LAPACK driver dsyev, which
+scipy.linalg.eigh(..., lower=False, driver='ev') calls.
+"""
+import numpy as np
+import scipy.linalg as sla
+
+from .exceptions import ZeroRankError
+
+__all__ = ["sort_by_norm", "spd_eig", "spd_inv_split", "eig_corr"]
+
+
+def sort_by_norm(evals, evecs):
+    """direct.py:7-32: descending |eigenvalue|."""
+    I = np.argsort(np.abs(evals))[::-1]
+    return evals[I], evecs[:, I]
+
+
+def _eig_qr(W):
+    """eig_qr.pyx:4-61 (dsytrd/dorgtr/dsteqr on 'U' == dsyev)."""
+    W = np.asarray(W, dtype=np.float64)
+    if W.size == 1:
+        return W.reshape(1) * np.ones(1), np.ones((1, 1))
+    return sla.eigh(W, lower=False, driver="ev")
+
+
+def spd_eig(W, epsilon=1e-10, method="QR", canonical_signs=False):
+    """direct.py:35-100: rank-truncated eigendecomposition of an SPD matrix."""
+    assert np.allclose(W.T, W), "W is not a symmetric matrix"
+    if method.lower() == "qr":
+        s, V = _eig_qr(W)
+    elif method.lower() == "schur":
+        S, V = sla.schur(W)
+        s = np.diag(S)
+    else:
+        raise ValueError("method not implemented: " + method)
+    s, V = sort_by_norm(s, V)
+    evmin = np.min(s)
+    if evmin < 0:
+        epsilon = max(epsilon, -evmin + 1e-16)
+    evnorms = np.abs(s)
+    n = np.shape(evnorms)[0]
+    m = n - np.searchsorted(evnorms[::-1], epsilon)
+    if m == 0:
+        raise ZeroRankError("All eigenvalues are smaller than %g, rank reduction would discard all dimensions."
+                            % epsilon)
+    Vm = V[:, 0:m]
+    sm = s[0:m]
+    if canonical_signs:
+        for j in range(m):
+            jj = np.argmax(np.abs(Vm[:, j]))
+            Vm[:, j] *= np.sign(Vm[jj, j])
+    return sm, Vm
+
+
+def spd_inv_split(W, epsilon=1e-10, method="QR", canonical_signs=False):
+    """direct.py:184-222: L with W^-1 = L L^T after rank truncation."""
+    if np.shape(W)[0] == 1:
+        if W[0, 0] < epsilon:
+            raise ZeroRankError("All eigenvalues are smaller than %g, rank reduction would discard all dimensions."
+                                % epsilon)
+        return 1.0 / np.sqrt(W[0, 0])
+    sm, Vm = spd_eig(W, epsilon=epsilon, method=method, canonical_signs=canonical_signs)
+    return np.dot(Vm, np.diag(1.0 / np.sqrt(sm)))
+
+
+def eig_corr(C0, Ct, epsilon=1e-10, method="QR", sign_maxelement=False):
+    """direct.py:225-295: solve Ct r = C0 r l through the whitened problem."""
+    L = spd_inv_split(C0, epsilon=epsilon, method=method, canonical_signs=True)
+    L = np.atleast_2d(L)
+    Ct_trans = np.dot(np.dot(L.T, Ct), L)
+    if np.allclose(Ct.T, Ct):
+        l, R_trans = sla.eigh(Ct_trans)
+    else:
+        l, R_trans = sla.eig(Ct_trans)
+    l, R_trans = sort_by_norm(l, R_trans)
+    R = np.dot(L, R_trans)
+    if sign_maxelement:
+        for j in range(R.shape[1]):
+            imax = np.argmax(np.abs(R[:, j]))
+            R[:, j] *= np.sign(R[imax, j])
+    return l, R

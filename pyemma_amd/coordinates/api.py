@@ -1,0 +1,105 @@
+"""User API with the reference's signatures (pyemma/coordinates/api.py).
+
+tica             api.py:1046-1290
+covariance_lagged api.py:1554-1661
+cluster_kmeans   api.py:1727-1866
+assign_to_centers api.py:2060-2156
+"""
+import types
+import warnings
+
+import numpy as np
+
+from .clustering import AssignCenters, KmeansClustering
+from .covariance import LaggedCovariance
+from .tica import TICA
+
+__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers"]
+
+
+def _weights_arg(weights, data, reversible=True, for_tica=True):
+    """api.py:1252-1272 / 1634-1652: only the empirical (unweighted) estimator runs on the GPU path."""
+    if isinstance(weights, str):
+        if weights == "koopman":
+            if data is None:
+                raise ValueError("Data must be supplied for reweighting='koopman'")
+            if for_tica and not reversible:
+                raise ValueError("Koopman re-weighting is designed for reversible processes, set reversible=True")
+            raise NotImplementedError("weights='koopman' is not implemented on the GPU path yet")
+        if weights == "empirical":
+            return None
+        raise ValueError("reweighting must be either 'empirical', 'koopman' or an object with a weights(data) "
+                         "method.")
+    if hasattr(weights, "weights") and isinstance(getattr(weights, "weights"), types.MethodType):
+        raise NotImplementedError("weight objects are not implemented on the GPU path yet")
+    if isinstance(weights, (list, tuple, np.ndarray)):
+        if for_tica and data is not None and len(data) != len(weights):
+            raise ValueError("len of weights({}) must match len of data({}).".format(len(weights), len(data)))
+        raise NotImplementedError("per-frame weights are not implemented on the GPU path yet")
+    raise ValueError("reweighting must be either 'empirical', 'koopman' or an object with a weights(data) method.")
+
+
+def tica(data=None, lag=10, dim=-1, var_cutoff=0.95, kinetic_map=True, commute_map=False, weights="empirical",
+         stride=1, remove_mean=True, skip=0, reversible=True, ncov_max=float("inf"), chunksize=None, **kwargs):
+    r"""Time-lagged independent component analysis (api.py:1046-1290)."""
+    weights = _weights_arg(weights, data, reversible=reversible, for_tica=True)
+    if not remove_mean:
+        warnings.warn("remove_mean option is deprecated. The mean is removed from the data by default, otherwise "
+                      "it cannot be guaranteed that all eigenvalues will be smaller than one.", DeprecationWarning)
+    precision = kwargs.pop("precision", "bf16x4")
+    res = TICA(lag, dim=dim, var_cutoff=var_cutoff, kinetic_map=kinetic_map, commute_map=commute_map, skip=skip,
+               stride=stride, weights=weights, reversible=reversible, ncov_max=ncov_max, precision=precision)
+    if data is not None:
+        res.estimate(data, chunksize=chunksize)
+    else:
+        res.chunksize = chunksize
+    return res
+
+
+def covariance_lagged(data=None, c00=True, c0t=True, ctt=False, remove_constant_mean=None, remove_data_mean=False,
+                      reversible=False, bessel=True, lag=0, weights="empirical", stride=1, skip=0, chunksize=None,
+                      ncov_max=float("inf"), column_selection=None, diag_only=False, **kwargs):
+    r"""Lagged covariances between time series (api.py:1554-1661)."""
+    weights = _weights_arg(weights, data, for_tica=False)
+    lc = LaggedCovariance(c00=c00, c0t=c0t, ctt=ctt, remove_constant_mean=remove_constant_mean,
+                          remove_data_mean=remove_data_mean, reversible=reversible, bessel=bessel, lag=lag,
+                          weights=weights, stride=stride, skip=skip, ncov_max=ncov_max,
+                          column_selection=column_selection, diag_only=diag_only,
+                          precision=kwargs.pop("precision", "bf16x4"))
+    if data is not None:
+        lc.estimate(data, chunksize=chunksize)
+    else:
+        lc.chunksize = chunksize
+    return lc
+
+
+def cluster_kmeans(data=None, k=None, max_iter=10, tolerance=1e-5, stride=1, metric="euclidean",
+                   init_strategy="kmeans++", fixed_seed=False, n_jobs=None, chunksize=None, skip=0, keep_data=False,
+                   clustercenters=None, **kwargs):
+    r"""k-means clustering (api.py:1727-1866)."""
+    res = KmeansClustering(n_clusters=k, max_iter=max_iter, metric=metric, tolerance=tolerance,
+                           init_strategy=init_strategy, fixed_seed=fixed_seed, n_jobs=n_jobs, skip=skip,
+                           keep_data=keep_data, clustercenters=clustercenters, stride=stride,
+                           assign_mode=kwargs.pop("assign_mode", "bf16x3"))
+    if data is not None:
+        res.estimate(data, chunksize=chunksize)
+    else:
+        res.chunksize = chunksize
+    return res
+
+
+def assign_to_centers(data=None, centers=None, stride=1, return_dtrajs=True, metric="euclidean", n_jobs=None,
+                      chunksize=None, skip=0, **kwargs):
+    r"""Assign data to the nearest cluster centers (api.py:2060-2156)."""
+    if centers is None:
+        raise ValueError("You have to provide centers in form of a filename or NumPy array or a reader created by "
+                         "source function")
+    res = AssignCenters(centers, metric=metric, n_jobs=n_jobs, skip=skip, stride=stride,
+                        assign_mode=kwargs.pop("assign_mode", "bf16x3"))
+    if data is not None:
+        res.estimate(data, chunksize=chunksize)
+        if return_dtrajs:
+            return res.dtrajs
+    else:
+        res.chunksize = chunksize
+    return res

@@ -1,0 +1,575 @@
+// Assignment of frames to their nearest cluster center.
+//
+// Reference semantics: label(x) = argmin_j sum_i (x_i - c_ji)^2 with the sum
+// evaluated in FP32, in index order, without fused multiply-add, strict '<'
+// (the first minimum wins) -- ClusterModel.transform as called from
+// coordinates/clustering/interface.py:157-167.
+//
+// Two device paths produce exactly these labels:
+//
+//  * k_assign_plain: the sequential FP32 loop itself on the vector ALUs
+//    (4 frames x 4 centers per thread, LDS-staged tiles).
+//  * k_assign_mfma + k_recheck: distances from bf16 hi/lo MFMA products on
+//    shifted data, d~_j = |a|^2 + |b_j|^2 - 2 a.b_j with a = fl(x - mu),
+//    b_j = fl(c_j - mu).  With E_j = CE (|a| + |b_j|)^2 a rigorous bound on
+//    |d~_j - d_j| (CE from u = 2^-24, the bf16 split residual 2^-16 and the
+//    FP32 accumulation of 3 d products, see assign_error_coeff) and
+//    g = gamma_{d+2} the relative rounding of the reference loop, a frame is
+//    decided when
+//        (1 - g) min_{j != best} (d~_j - E_j)  >  (1 + g)(d~_best + E_best);
+//    then no other center can reach the reference loop's minimum.  All other
+//    frames are appended to a list and re-evaluated by k_recheck with the
+//    sequential FP32 loop over every center.
+#include <math.h>
+
+#include "emma_common.hpp"
+
+namespace emma {
+namespace asg {
+
+// ------------------------------------------------------------------
+// exact sequential FP32 distance pieces (library built with -ffp-contract=off)
+// ------------------------------------------------------------------
+__device__ __forceinline__ float sq_step(float acc, float x, float c) {
+    const float t = __fsub_rn(x, c);
+    return __fadd_rn(acc, __fmul_rn(t, t));
+}
+
+// ------------------------------------------------------------------
+// plain kernel: 64 frames x all centers per workgroup
+// ------------------------------------------------------------------
+constexpr int PF = 64;      // frames per workgroup
+constexpr int PC = 64;      // centers per tile
+constexpr int PD = 32;      // dims per staged chunk
+constexpr int PPITCH = 68;  // floats per staged row (16-byte aligned)
+
+__global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ X, int64_t n, int d,
+                                                      const float *__restrict__ C, int k,
+                                                      int32_t *__restrict__ labels,
+                                                      const int64_t *__restrict__ list,
+                                                      const int64_t *__restrict__ list_count,
+                                                      const int32_t *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ __align__(16) float xs[PD][PPITCH];
+    __shared__ __align__(16) float cs[PD][PPITCH];
+    __shared__ float s_best[16][PF];
+    __shared__ int s_bidx[16][PF];
+    __shared__ int64_t s_row[PF];
+
+    const int tid = threadIdx.x;
+    const int fg = tid >> 4;  // frames fg*4 .. +3
+    const int cg = tid & 15;  // centers cg*4 .. +3 within the tile
+    const int64_t f0 = (int64_t)blockIdx.x * PF;
+    const int64_t nrows = list ? *list_count : n;
+    if (f0 >= nrows) return;
+    if (tid < PF) {
+        const int64_t f = f0 + tid;
+        s_row[tid] = f < nrows ? (list ? list[f] : f) : -1;
+    }
+    __syncthreads();
+
+    float best[4];
+    int bidx[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        best[a] = INFINITY;
+        bidx[a] = 0x7fffffff;
+    }
+    // loader mapping: row r = tid / 4 (0..63), dims (tid % 4) * 8 .. +7
+    const int lr = tid >> 2, ld0 = (tid & 3) * 8;
+    const int64_t xrow = s_row[lr];
+
+    for (int c0 = 0; c0 < k; c0 += PC) {
+        float acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+        const int crow = c0 + lr;
+        for (int i0 = 0; i0 < d; i0 += PD) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int i = i0 + ld0 + q;
+                xs[ld0 + q][lr] = (xrow >= 0 && i < d) ? X[xrow * d + i] : 0.0f;
+                cs[ld0 + q][lr] = (crow < k && i < d) ? C[(int64_t)crow * d + i] : 0.0f;
+            }
+            __syncthreads();
+            const int ni = d - i0 < PD ? d - i0 : PD;
+            for (int ii = 0; ii < ni; ++ii) {
+                const float4 xv = *reinterpret_cast<const float4 *>(&xs[ii][fg * 4]);
+                const float4 cv = *reinterpret_cast<const float4 *>(&cs[ii][cg * 4]);
+                const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+                const float ca[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = sq_step(acc[a][b], xa[a], ca[b]);
+            }
+        }
+        // centers of this thread in increasing index order: strict '<'
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = c0 + cg * 4 + b;
+            if (j < k) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+                    if (acc[a][b] < best[a]) {
+                        best[a] = acc[a][b];
+                        bidx[a] = j;
+                    }
+            }
+        }
+    }
+    // reduce over the 16 center groups: smallest distance, then smallest index
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        s_best[cg][fg * 4 + a] = best[a];
+        s_bidx[cg][fg * 4 + a] = bidx[a];
+    }
+    __syncthreads();
+    if (tid < PF) {
+        float bv = s_best[0][tid];
+        int bi = s_bidx[0][tid];
+        for (int g = 1; g < 16; ++g) {
+            const float v = s_best[g][tid];
+            const int vi = s_bidx[g][tid];
+            if (v < bv || (v == bv && vi < bi)) {
+                bv = v;
+                bi = vi;
+            }
+        }
+        if (s_row[tid] >= 0) labels[s_row[tid]] = bi;
+    }
+}
+
+// ------------------------------------------------------------------
+// exact recheck: one wavefront per listed frame, lanes split the centers
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_recheck(const float *__restrict__ X, int d, const float *__restrict__ C,
+                                                 int k, int32_t *__restrict__ labels,
+                                                 const int64_t *__restrict__ list,
+                                                 const int64_t *__restrict__ list_count,
+                                                 const int32_t *__restrict__ done) {
+    if (done && *done) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t cnt = *list_count;
+    const int64_t stride = (int64_t)gridDim.x * 4;
+    for (int64_t e = (int64_t)blockIdx.x * 4 + wave; e < cnt; e += stride) {
+        const int64_t row = list[e];
+        const float *x = X + row * d;  // every lane reads the same row: cache broadcast
+        float best = INFINITY;
+        int bi = 0x7fffffff;
+        for (int j = lane; j < k; j += 64) {
+            const float *c = C + (int64_t)j * d;
+            float acc = 0.0f;
+            for (int i = 0; i < d; ++i) acc = sq_step(acc, x[i], c[i]);
+            if (acc < best) {
+                best = acc;
+                bi = j;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float ov = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov < best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) labels[row] = bi;
+    }
+}
+
+__global__ void k_copy_i64(int64_t *dst, const int64_t *src) { *dst = *src; }
+
+// ------------------------------------------------------------------
+// center preparation for the MFMA path
+// ------------------------------------------------------------------
+struct CenterPack {
+    uint16_t *hi, *lo;  // [k_pad][d_pad] bf16 bits of b = fl(c - mu)
+    float *nb;          // |b|^2 (FP32 sequential); +inf for padding centers
+    float *q;           // nb - CE * bn^2
+    float *r;           // 2 * CE * bn
+    float *bn;          // |b| rounded up
+    float *mu;          // [d_pad] shift
+};
+
+// mu = mean of the centers (FP64, rounded to FP32)
+__global__ __launch_bounds__(256) void k_center_mean(const float *__restrict__ C, int k, int d, int d_pad,
+                                                     float *__restrict__ mu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d_pad) return;
+    if (i >= d) {
+        mu[i] = 0.0f;
+        return;
+    }
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += (double)C[(int64_t)j * d + i];
+    mu[i] = (float)(s / k);
+}
+
+// one wavefront per (padded) center
+__global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C, int k, int d, int d_pad,
+                                                     int k_pad, float CE, CenterPack cp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + wave;
+    if (j >= k_pad) return;
+    double n2 = 0.0;
+    for (int i0 = 0; i0 < d_pad; i0 += 64) {
+        const int i = i0 + lane;
+        float b = 0.0f;
+        if (j < k && i < d) b = __fsub_rn(C[(int64_t)j * d + i], cp.mu[i]);
+        uint32_t h, l;
+        split_bf16(b, h, l);
+        if (i < d_pad) {
+            cp.hi[(int64_t)j * d_pad + i] = (uint16_t)h;
+            cp.lo[(int64_t)j * d_pad + i] = (uint16_t)l;
+        }
+        n2 += (double)b * (double)b;
+    }
+    // FP32 |b|^2 in index order (one lane walks the row; the error bound only
+    // needs gamma_d, any order would do, this keeps it deterministic)
+    double n2s = wave_butterfly_sum(n2);
+    if (lane == 0) {
+        if (j < k) {
+            float acc = 0.0f;
+            for (int i = 0; i < d; ++i) {
+                const float b = __fsub_rn(C[(int64_t)j * d + i], cp.mu[i]);
+                acc = __fadd_rn(acc, __fmul_rn(b, b));
+            }
+            const float bn = (float)(sqrt(n2s) * (1.0 + 1e-6));  // rounded up
+            cp.nb[j] = acc;
+            cp.bn[j] = bn;
+            cp.q[j] = acc - CE * bn * bn;
+            cp.r[j] = 2.0f * CE * bn;
+        } else {
+            cp.nb[j] = INFINITY;
+            cp.bn[j] = 0.0f;
+            cp.q[j] = INFINITY;
+            cp.r[j] = 0.0f;
+        }
+    }
+}
+
+__global__ void k_zero_i64n(int64_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------
+// MFMA kernel
+// ------------------------------------------------------------------
+// Wave w of a workgroup owns 32 frames; their bf16 hi/lo fragments stay in
+// registers for all centers.  Centers stream through LDS in tiles of 32,
+// double-buffered with a register prefetch.  v_mfma_f32_32x32x16_bf16:
+// A = 32 centers x 16 dims, B = 16 dims x 32 frames; lane l holds frame
+// l % 32 and centers 8(e>>2) + 4(l>>5) + (e&3), e = 0..15.
+template <int KSTEPS>
+struct MfmaLds {
+    static constexpr int DP = KSTEPS * 16;     // padded dims
+    static constexpr int PITCH = DP + 8;       // uint16 per center row: 16-byte reads conflict-free
+    alignas(16) uint16_t a[2][2][32 * PITCH];  // [buffer][hi|lo][center row]
+    alignas(16) float cst[2][3][32];           // [buffer][nb|q|r][center]
+};
+
+template <int KSTEPS, int NW>
+__global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
+                                                         CenterPack cp, int k, int k_pad, float CE,
+                                                         float gref, int32_t *__restrict__ labels,
+                                                         int64_t *__restrict__ list,
+                                                         int64_t *__restrict__ list_count,
+                                                         const int32_t *__restrict__ done) {
+    if (done && *done) return;
+    using L = MfmaLds<KSTEPS>;
+    constexpr int DP = L::DP;
+    constexpr int NT = NW * 64;
+    constexpr int CHUNKS = 32 * DP / 8;                    // 16-byte chunks per (tile, hi|lo)
+    constexpr int PER_T = (2 * CHUNKS + NT - 1) / NT;      // prefetch uint4 per thread
+    __shared__ L lds;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t frame = ((int64_t)blockIdx.x * NW + wave) * 32 + (lane & 31);
+    const bool fvalid = frame < n;
+    const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
+
+    // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo ----
+    bf16x8 bhi[KSTEPS], blo[KSTEPS];
+    float na_part = 0.0f;
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) {
+        uint32_t h[8], l[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = s * 16 + kh + q;
+            float a = 0.0f;
+            if (fvalid && i < d) a = __fsub_rn(X[frame * d + i], cp.mu[i]);
+            na_part = __fadd_rn(na_part, __fmul_rn(a, a));
+            split_bf16(a, h[q], l[q]);
+        }
+        uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
+        uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
+        bhi[s] = __builtin_bit_cast(bf16x8, hv);
+        blo[s] = __builtin_bit_cast(bf16x8, lv);
+    }
+    const float na = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
+    const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
+
+    // ---- center tile staging ----
+    const int ntiles = k_pad / 32;
+    uint4 pre[PER_T];
+#define EMMA_FETCH(t_)                                                                            \
+    _Pragma("unroll") for (int u = 0; u < PER_T; ++u) {                                          \
+        const int c = tid + u * NT;                                                               \
+        if (c < 2 * CHUNKS) {                                                                     \
+            const int part = c / CHUNKS, cc = c - part * CHUNKS;                                  \
+            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);                            \
+            const uint16_t *src = (part == 0 ? cp.hi : cp.lo) + ((int64_t)((t_) * 32 + row)) * DP + col8 * 8; \
+            pre[u] = *reinterpret_cast<const uint4 *>(src);                                       \
+        }                                                                                         \
+    }
+#define EMMA_STORE(t_, buf_)                                                                      \
+    _Pragma("unroll") for (int u = 0; u < PER_T; ++u) {                                          \
+        const int c = tid + u * NT;                                                               \
+        if (c < 2 * CHUNKS) {                                                                     \
+            const int part = c / CHUNKS, cc = c - part * CHUNKS;                                  \
+            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);                            \
+            *reinterpret_cast<uint4 *>(&lds.a[(buf_)][part][row * L::PITCH + col8 * 8]) = pre[u]; \
+        }                                                                                         \
+    }                                                                                             \
+    if (tid < 96) {                                                                               \
+        const int which = tid >> 5, c = tid & 31;                                                 \
+        const float *srcv = which == 0 ? cp.nb : (which == 1 ? cp.q : cp.r);                      \
+        lds.cst[(buf_)][which][c] = srcv[(t_) * 32 + c];                                          \
+    }
+
+    float tmin = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
+    int imin = 0x7fffffff, ilb1 = 0x7fffffff;
+
+    EMMA_FETCH(0)
+    EMMA_STORE(0, 0)
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) { EMMA_FETCH(t + 1) }
+        floatx16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        const uint16_t *ah_base = &lds.a[buf][0][(lane & 31) * L::PITCH + kh];
+        const uint16_t *al_base = &lds.a[buf][1][(lane & 31) * L::PITCH + kh];
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base + s * 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo[s], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[s], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[s], acc, 0, 0, 0);
+        }
+        // epilogue: running best d~ and the two smallest lower bounds
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int c0 = 8 * g + 4 * (lane >> 5);
+            const float4 nbv = *reinterpret_cast<const float4 *>(&lds.cst[buf][0][c0]);
+            const float4 qv = *reinterpret_cast<const float4 *>(&lds.cst[buf][1][c0]);
+            const float4 rv = *reinterpret_cast<const float4 *>(&lds.cst[buf][2][c0]);
+            const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
+            const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+            const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const float p = acc[4 * g + h];
+                const int j = t * 32 + c0 + h;
+                const float tv = fmaf(-2.0f, p, nba[h]);
+                const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
+                if (tv < tmin) {
+                    tmin = tv;
+                    imin = j;
+                }
+                if (lb < lb1) {
+                    lb2 = lb1;
+                    lb1 = lb;
+                    ilb1 = j;
+                } else if (lb < lb2) {
+                    lb2 = lb;
+                }
+            }
+        }
+        if (t + 1 < ntiles) { EMMA_STORE(t + 1, buf ^ 1) }
+        __syncthreads();
+    }
+
+#undef EMMA_FETCH
+#undef EMMA_STORE
+    // ---- merge the two half-waves that hold the same frame ----
+    {
+        const float ot = __shfl_xor(tmin, 32, 64);
+        const int oi = __shfl_xor(imin, 32, 64);
+        const float o1 = __shfl_xor(lb1, 32, 64);
+        const int oi1 = __shfl_xor(ilb1, 32, 64);
+        const float o2 = __shfl_xor(lb2, 32, 64);
+        if (ot < tmin || (ot == tmin && oi < imin)) {
+            tmin = ot;
+            imin = oi;
+        }
+        if (o1 < lb1) {
+            lb2 = fminf(lb1, o2);
+            lb1 = o1;
+            ilb1 = oi1;
+        } else {
+            lb2 = fminf(lb2, o1);
+        }
+    }
+    if (lane < 32 && fvalid && imin >= k) {
+        // non-finite distances: leave it to the exact loop
+        labels[frame] = 0;
+        const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
+        list[pos] = frame;
+    } else if (lane < 32 && fvalid) {
+        // decision in FP64
+        const double lbo = (ilb1 == imin) ? (double)lb2 : (double)lb1;
+        const double bnb = (double)cp.bn[imin];
+        const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
+        const double upper = ((double)na + (double)tmin + E1) * (1.0 + (double)gref);
+        const double lower = ((double)na - (double)CE * (double)sa * (double)sa + lbo) * (1.0 - (double)gref);
+        labels[frame] = imin;
+        if (!(lower > upper)) {
+            const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
+            list[pos] = frame;
+        }
+    }
+}
+
+// Coefficient CE of the per-candidate bound E_j = CE (|a| + |b_j|)^2 for
+// |d~_j - d_j|, with u = 2^-24, gamma_n = n u / (1 - n u):
+//   shift rounding of a and b:            2.1 u (|a|+|b|)^2
+//   FP32 norms |a|^2, |b|^2:              gamma_d (|a|^2 + |b|^2)
+//   omitted bf16 split terms:             3.1 * 2^-16 * 2 sum|a_i b_i|
+//   FP32 MFMA accumulation of 3 d_pad terms: 1.02 gamma_{3 d_pad} * 2 sum|a_i b_i|
+//   final combination and fmaf rounding:  2.1 u (|a|+|b|)^2
+// using 2 sum|a_i b_i| <= (|a|+|b|)^2.  A 25 % margin covers the rounding of
+// the FP32 bound arithmetic itself.
+static double assign_error_coeff(int d, int d_pad) {
+    const double u = ldexp(1.0, -24);
+    auto gam = [&](double m) { return m * u / (1.0 - m * u); };
+    const double c = 4.2 * u + gam(d) + 3.1 * ldexp(1.0, -16) + 1.02 * gam(3.0 * d_pad);
+    return 1.25 * c;
+}
+
+static double gamma_ref(int d) {
+    const double u = ldexp(1.0, -24);
+    const double m = d + 2;
+    return 1.25 * m * u / (1.0 - m * u);
+}
+
+static int pick_ksteps(int d) {
+    const int ks = (int)cdiv(d, 16);
+    static const int sizes[] = {2, 4, 8, 12, 16, 20, 24, 32};
+    for (int s : sizes)
+        if (ks <= s) return s;
+    return -1;
+}
+
+struct AssignWs {
+    CenterPack cp;
+    int64_t *list;
+    int64_t *count;
+    size_t bytes;
+};
+
+static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
+    AssignWs w{};
+    Carver cv(ws);
+    w.count = cv.take<int64_t>(1);
+    w.list = cv.take<int64_t>(n > 0 ? (size_t)n : 1);
+    if (mode == EMMA_ASSIGN_BF16X3_RECHECK) {
+        const int ks = pick_ksteps(d);
+        const int d_pad = (ks > 0 ? ks : 1) * 16;
+        const int k_pad = (int)cdiv(k, 32) * 32;
+        w.cp.hi = cv.take<uint16_t>((size_t)k_pad * d_pad);
+        w.cp.lo = cv.take<uint16_t>((size_t)k_pad * d_pad);
+        w.cp.nb = cv.take<float>(k_pad);
+        w.cp.q = cv.take<float>(k_pad);
+        w.cp.r = cv.take<float>(k_pad);
+        w.cp.bn = cv.take<float>(k_pad);
+        w.cp.mu = cv.take<float>(d_pad);
+    }
+    w.bytes = cv.bytes();
+    return w;
+}
+
+template <int KS>
+static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
+                           float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
+    // 4 waves per workgroup: 1-2 waves per SIMD given the fragment registers
+    constexpr int NW = 4;
+    const int64_t blocks = cdiv(n, 32 * NW);
+    k_assign_mfma<KS, NW><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels, w.list,
+                                                               w.count, done);
+}
+
+}  // namespace asg
+
+// shared with kmeans.hip (Lloyd)
+size_t assign_workspace_bytes(int64_t n, int d, int k, int mode) { return asg::carve_assign(nullptr, n, d, k, mode).bytes; }
+
+int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t *labels, int mode,
+               int64_t *n_rechecked, void *workspace, const int32_t *done, hipStream_t s) {
+    using namespace asg;
+    AssignWs w = carve_assign(workspace, n, d, k, mode);
+    if (mode == EMMA_ASSIGN_FP32_PLAIN) {
+        k_assign_plain<<<(unsigned)cdiv(n, PF), 256, 0, s>>>(X, n, d, C, k, labels, nullptr, nullptr, done);
+        EMMA_TRY(check_launch("assign plain"));
+        if (n_rechecked) k_zero_i64n<<<1, 64, 0, s>>>(n_rechecked, 1);
+        return check_launch("assign plain");
+    }
+    const int ks = pick_ksteps(d);
+    EMMA_REQUIRE(ks > 0, "emma_assign: d=%d exceeds the MFMA path limit (512)", d);
+    const int d_pad = ks * 16;
+    const int k_pad = (int)cdiv(k, 32) * 32;
+    const float CE = (float)assign_error_coeff(d, d_pad);
+    const float gref = (float)gamma_ref(d);
+    k_zero_i64n<<<1, 64, 0, s>>>(w.count, 1);
+    k_center_mean<<<(unsigned)cdiv(d_pad, 256), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
+    k_center_pack<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, CE, w.cp);
+    EMMA_TRY(check_launch("assign center pack"));
+    switch (ks) {
+        case 2: launch_mfma_ks<2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 4: launch_mfma_ks<4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 8: launch_mfma_ks<8>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 12: launch_mfma_ks<12>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 16: launch_mfma_ks<16>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 20: launch_mfma_ks<20>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 24: launch_mfma_ks<24>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+    }
+    EMMA_TRY(check_launch("assign mfma"));
+    // recheck: enough waves to cover the whole device; each loops over the list
+    k_recheck<<<1024, 256, 0, s>>>(X, d, C, k, labels, w.list, w.count, done);
+    EMMA_TRY(check_launch("assign recheck"));
+    if (n_rechecked) k_copy_i64<<<1, 1, 0, s>>>(n_rechecked, w.count);
+    return check_launch("assign copy count");
+}
+
+}  // namespace emma
+
+using namespace emma;
+
+extern "C" size_t emma_assign_workspace_size(int64_t n, int32_t d, int32_t k, int32_t mode) {
+    if (n < 0 || d <= 0 || k <= 0) return 0;
+    return assign_workspace_bytes(n, d, k, mode);
+}
+
+extern "C" int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k, int32_t *labels,
+                           int32_t mode, int64_t *n_rechecked, void *workspace, size_t workspace_bytes,
+                           void *stream) {
+    EMMA_REQUIRE(n >= 0 && d > 0 && k > 0, "emma_assign: bad sizes n=%lld d=%d k=%d", (long long)n, d, k);
+    EMMA_REQUIRE(mode == EMMA_ASSIGN_FP32_PLAIN || mode == EMMA_ASSIGN_BF16X3_RECHECK,
+                 "emma_assign: unknown mode %d", mode);
+    if (n == 0) return EMMA_OK;
+    EMMA_REQUIRE(X && C && labels, "emma_assign: NULL pointer");
+    const size_t need = assign_workspace_bytes(n, d, k, mode);
+    EMMA_REQUIRE(workspace && workspace_bytes >= need, "emma_assign: workspace too small (%zu < %zu)",
+                 workspace_bytes, need);
+    return assign_run(X, n, d, C, k, labels, mode, n_rechecked, workspace, nullptr, (hipStream_t)stream);
+}

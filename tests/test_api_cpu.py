@@ -1,0 +1,102 @@
+"""Host-side API semantics that need no GPU (argument checks mirror the reference)."""
+import numpy as np
+import pytest
+
+from pyemma_amd.coordinates import api
+from pyemma_amd.coordinates.clustering import KmeansClustering, get_n_jobs
+from pyemma_amd.coordinates.covariance import LaggedCovariance, Moments, MomentsStorage
+from pyemma_amd.coordinates.tica import TICA
+
+
+def test_tica_parameter_validation():
+    """test_tica.py:283-300."""
+    with pytest.raises(ValueError):
+        TICA(10, dim=1, var_cutoff=0.9)
+    with pytest.raises(ValueError):
+        api.tica(lag=10, var_cutoff=0)
+    with pytest.raises(ValueError):
+        api.tica(lag=10, var_cutoff=1.1)
+    with pytest.raises(ValueError):
+        TICA(10, kinetic_map=True, commute_map=True)
+    t = TICA(5, dim=3)
+    assert t.dimension() == 3
+    assert "TICA, lag = 5" in t.describe()
+    with pytest.warns(UserWarning):
+        t = TICA(5, reversible=False)
+    assert not t.kinetic_map
+
+
+def test_covariance_validation():
+    with pytest.raises(ValueError):
+        LaggedCovariance(c0t=True, lag=0)
+    with pytest.raises(ValueError):
+        LaggedCovariance(remove_constant_mean=np.zeros(2), remove_data_mean=True)
+    with pytest.raises(ValueError):
+        LaggedCovariance(column_selection=np.array([0]), diag_only=True)
+    with pytest.raises(ValueError):
+        LaggedCovariance(c0t=True, ctt=True, lag=1, reversible=True)
+    with pytest.raises(NotImplementedError):
+        api.covariance_lagged(np.zeros((10, 2)), lag=1, weights=np.ones(10))
+
+
+def test_weights_arguments():
+    with pytest.raises(ValueError):
+        api.tica(np.zeros((10, 2)), weights="nonsense")
+    with pytest.raises(ValueError):
+        api.tica(lag=1, weights="koopman")
+    with pytest.raises(ValueError):
+        api.tica(np.zeros((10, 2)), lag=1, weights="koopman", reversible=False)
+
+
+def test_assign_and_kmeans_validation():
+    with pytest.raises(ValueError):
+        api.assign_to_centers(np.zeros((10, 3)), centers=None)
+    with pytest.raises(ValueError):
+        api.assign_to_centers(np.zeros((10, 3)), centers=np.zeros(3))
+    with pytest.raises(ValueError):
+        KmeansClustering(3, init_strategy="foo")
+    with pytest.raises(NotImplementedError):
+        KmeansClustering(3, metric="minRMSD")
+
+
+def test_fixed_seed_semantics():
+    """kmeans.py:142-164, test_kmeans.py:130-137."""
+    assert KmeansClustering(2, fixed_seed=True).fixed_seed == 42
+    assert 0 <= KmeansClustering(2, fixed_seed=-1).fixed_seed < 2 ** 32
+    assert KmeansClustering(2, fixed_seed=2 ** 32).fixed_seed < 2 ** 32
+    assert KmeansClustering(2, fixed_seed=463498).fixed_seed == 463498
+    with pytest.raises(ValueError):
+        KmeansClustering(2, fixed_seed=1.5)
+
+
+def test_n_jobs_resolution(monkeypatch):
+    """_base/parallel.py:2-38 and test_assign.py:199-230."""
+    monkeypatch.setenv("PYEMMA_NJOBS", "3")
+    monkeypatch.delenv("SLURM_CPUS_ON_NODE", raising=False)
+    assert get_n_jobs() == 3
+    monkeypatch.setenv("SLURM_CPUS_ON_NODE", "5")
+    assert get_n_jobs() == 5
+    monkeypatch.delenv("SLURM_CPUS_ON_NODE")
+    monkeypatch.setenv("PYEMMA_NJOBS", "this is not right")
+    import psutil
+    monkeypatch.setattr(psutil, "cpu_count", lambda *a, **k: 7)
+    assert get_n_jobs() == 7
+    assert KmeansClustering(2, n_jobs=2).n_jobs == 2
+
+
+def test_host_chan_storage_matches_single_pass():
+    """The host merge (running_moments.py:35-148) of split moments equals one pass."""
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((3000, 3)) + 5
+
+    def mom(A):
+        s = A.sum(0)
+        A0 = A - s / len(A)
+        return Moments(len(A), s, s, A0.T @ A0)
+    st = MomentsStorage(3, remove_mean=True)
+    for lo in range(0, 3000, 250):
+        st.store(mom(X[lo:lo + 250]))
+    m = st.moments
+    ref = mom(X)
+    np.testing.assert_allclose(m.Mxy, ref.Mxy, rtol=1e-11)
+    np.testing.assert_allclose(m.sx, ref.sx, rtol=1e-13)

@@ -1,0 +1,89 @@
+"""GPU assignment through the C ABI vs the sequential-FP32 oracle: bit-exact labels."""
+import numpy as np
+import pytest
+
+from oracle import native as on
+from pyemma_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+MODES = [_native.EMMA_ASSIGN_FP32_PLAIN, _native.EMMA_ASSIGN_BF16X3_RECHECK]
+
+
+def _run(X, C, mode):
+    t = _native.torch()
+    Xt = _native.as_device_f32(X)
+    Ct = _native.as_device_f32(C)
+    nre = t.zeros(1, dtype=t.int64, device=Xt.device)
+    lab = _native.assign(Xt, Ct, mode=mode, n_rechecked=nre)
+    t.cuda.synchronize()
+    return lab.cpu().numpy(), int(nre.item())
+
+
+def _data(n, d, k, seed, offset=0.0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((n, d)) * scale + offset).astype(np.float32)
+    C = X[rng.choice(n, k, replace=n < k)] + (rng.standard_normal((k, d)) * 0.3 * scale).astype(np.float32)
+    return X, C.astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,d,k", [(1000, 30, 100), (2049, 300, 257), (777, 3, 1), (1500, 1, 5),
+                                   (3000, 128, 512), (1024, 512, 64), (513, 17, 33), (300, 64, 1000)])
+def test_labels_bit_exact(mode, n, d, k):
+    X, C = _data(n, d, k, seed=n + d + k)
+    ref = on.assign(X, C, nthreads=4)
+    lab, nre = _run(X, C, mode)
+    np.testing.assert_array_equal(lab, ref)
+    if mode == _native.EMMA_ASSIGN_FP32_PLAIN:
+        assert nre == 0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_far_from_origin_and_large_spread(mode):
+    """Shifted data (mean 1e3) and mixed scales: the shift and the bound must hold."""
+    X, C = _data(4000, 40, 80, seed=5, offset=1000.0, scale=0.5)
+    np.testing.assert_array_equal(_run(X, C, mode)[0], on.assign(X, C, nthreads=4))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_exact_ties_first_index_wins(mode):
+    rng = np.random.default_rng(1)
+    C = rng.standard_normal((10, 8)).astype(np.float32)
+    C = np.concatenate([C, C[::-1]], axis=0)  # every center twice
+    X = np.concatenate([C, rng.standard_normal((500, 8)).astype(np.float32)], axis=0)
+    ref = on.assign(X, C)
+    lab, nre = _run(X, C, mode)
+    np.testing.assert_array_equal(lab, ref)
+    if mode == _native.EMMA_ASSIGN_BF16X3_RECHECK:
+        assert nre >= 20  # duplicated centers always tie
+
+
+def test_golden_hmm_labels(golden_hmm):
+    g = golden_hmm
+    for mode in MODES:
+        np.testing.assert_array_equal(_run(g["X"], g["lloyd_centers"], mode)[0], g["labels"])
+
+
+def test_recheck_rate_is_small_on_separated_data():
+    from pyemma_amd.synthetic import hmm_trajectory
+    X = hmm_trajectory(20000, 30, seed=3)
+    C = X[::200][:100].copy()
+    lab, nre = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+    np.testing.assert_array_equal(lab, on.assign(X, C, nthreads=4))
+    assert nre < 0.2 * len(X)
+
+
+def test_mfma_dimension_limit_and_plain_beyond():
+    X, C = _data(300, 600, 7, seed=2)
+    with pytest.raises(ValueError):
+        _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+    np.testing.assert_array_equal(_run(X, C, _native.EMMA_ASSIGN_FP32_PLAIN)[0], on.assign(X, C))
+
+
+def test_deterministic_repeat():
+    X, C = _data(5000, 300, 300, seed=9)
+    a, na = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+    b, nb = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+    np.testing.assert_array_equal(a, b)
+    assert na == nb

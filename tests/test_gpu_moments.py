@@ -1,0 +1,125 @@
+"""GPU lagged moments through the C ABI vs the FP64 oracle.
+
+Tolerance: the covariance tiles accumulate in FP32 over at most 1024 pairs
+and merge in FP64, so entries agree with the FP64 oracle to ~1e-6 of the
+matrix scale: rtol 2e-5 with atol 2e-6 * max|C|.
+"""
+import numpy as np
+import pytest
+
+from oracle import eig as oeig
+from oracle import moments as om
+from pyemma_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_FP32]
+
+
+def _gpu(trajs, lag, prec, **kw):
+    t = _native.torch()
+    X = np.concatenate([np.asarray(a, dtype=np.float32).reshape(len(a), -1) for a in trajs])
+    off = np.concatenate([[0], np.cumsum([len(a) for a in trajs])]).astype(np.int64)
+    Xt = _native.as_device_f32(X)
+    out = _native.moments_lagged(Xt, off, lag, precision=prec, **kw)
+    t.cuda.synchronize()
+    return [None if o is None else o.cpu().numpy() for o in out]
+
+
+def _close(a, b, rtol=2e-5, atol_rel=2e-6):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    np.testing.assert_allclose(a, b, rtol=rtol, atol=atol_rel * scale)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("d", [1, 3, 30, 130, 300])
+def test_reversible_mean_free(prec, d):
+    rng = np.random.default_rng(d)
+    trajs = [rng.standard_normal((T, d)).cumsum(0) * 0.1 + 3.0 for T in (3000, 57, 10, 1200)]
+    trajs = [t.astype(np.float32) for t in trajs]
+    lag = 10
+    ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=True, bessel=False)
+    w, sx, sy, Mxx, Mxy, _ = _gpu(trajs, lag, prec, reversible=True, remove_mean=True, c0t=True)
+    assert w[0] == ref["w"]
+    _close(sx / w[0], ref["mean"], rtol=1e-6)
+    _close(Mxx / w[0], ref["C00"])
+    _close(Mxy / w[0], ref["C0t"])
+    np.testing.assert_array_equal(Mxx, Mxx.T)
+    np.testing.assert_array_equal(Mxy, Mxy.T)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("remove_mean", [True, False])
+def test_nonreversible_with_ctt(prec, remove_mean):
+    rng = np.random.default_rng(4)
+    trajs = [(rng.standard_normal((T, 7)).cumsum(0) * 0.2 + 1.0).astype(np.float32) for T in (800, 333, 5)]
+    lag = 3
+    ref = om.covariance_lagged(trajs, lag, c00=True, c0t=True, ctt=True, remove_data_mean=remove_mean,
+                               reversible=False, bessel=False)
+    w, sx, sy, Mxx, Mxy, Myy = _gpu(trajs, lag, prec, reversible=False, remove_mean=remove_mean, c0t=True,
+                                   ctt=True)
+    assert w[0] == ref["w"]
+    _close(sx / w[0], ref["mean"], rtol=1e-6)
+    _close(sy / w[0], ref["mean_tau"], rtol=1e-6)
+    _close(Mxx / w[0], ref["C00"])
+    _close(Mxy / w[0], ref["C0t"])
+    _close(Myy / w[0], ref["Ctt"])
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_lag0_stride_skip(prec):
+    rng = np.random.default_rng(6)
+    trajs = [(rng.standard_normal((T, 5)) + 2).astype(np.float32) for T in (100, 37)]
+    ref = om.covariance_lagged(trajs, 0, c00=True, c0t=False, remove_data_mean=True, bessel=False,
+                               stride=3, skip=2)
+    w, sx, _, Mxx, _, _ = _gpu(trajs, 0, prec, stride=3, skip=2, reversible=False, remove_mean=True, c0t=False)
+    assert w[0] == ref["w"]
+    _close(Mxx / w[0], ref["C00"])
+    for lag, stride, skip in [(4, 3, 2), (1, 2, 5)]:
+        ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=True, bessel=False,
+                                   stride=stride, skip=skip)
+        w, sx, _, Mxx, Mxy, _ = _gpu(trajs, lag, prec, stride=stride, skip=skip, reversible=True,
+                                     remove_mean=True, c0t=True)
+        assert w[0] == ref["w"]
+        _close(Mxx / w[0], ref["C00"])
+        _close(Mxy / w[0], ref["C0t"])
+
+
+def test_koopman_fixture_moments(koopman_data):
+    """Inputs of the reference's Koopman tests (test_tica.py:443-452), lag 10."""
+    ref = om.covariance_lagged(koopman_data, 10, remove_data_mean=True, reversible=True, bessel=False)
+    w, sx, _, Mxx, Mxy, _ = _gpu(koopman_data, 10, _native.EMMA_PREC_BF16X4, reversible=True, remove_mean=True,
+                                 c0t=True)
+    assert w[0] == ref["w"]
+    _close(Mxx / w[0], ref["C00"], rtol=1e-6, atol_rel=1e-7)
+    _close(Mxy / w[0], ref["C0t"], rtol=1e-6, atol_rel=1e-7)
+
+
+def test_no_pairs_gives_zero_weight():
+    w, sx, _, Mxx, Mxy, _ = _gpu([np.ones((5, 3), np.float32)], 10, _native.EMMA_PREC_BF16X4, reversible=True,
+                                 remove_mean=True, c0t=True)
+    assert w[0] == 0.0 and np.all(Mxx == 0)
+
+
+def test_tica_eigenvalues_within_1e6_of_fp64():
+    """HMM data (config 1 generator, 100k x 30): eigenvalues vs the FP64 oracle."""
+    from pyemma_amd.synthetic import hmm_trajectory
+    X = hmm_trajectory(100000, 30, seed=1)
+    lag = 10
+    ref = om.covariance_lagged([X], lag, remove_data_mean=True, reversible=True, bessel=False)
+    ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
+    for prec in PRECS:
+        w, sx, _, Mxx, Mxy, _ = _gpu([X], lag, prec, reversible=True, remove_mean=True, c0t=True)
+        ev = oeig.generalized_eigenvalues(Mxx / w[0], Mxy / w[0])
+        rel = np.abs(ev - ev_ref) / np.abs(ev_ref)
+        assert rel.max() < 1e-6, (prec, rel.max())
+
+
+def test_deterministic_repeat():
+    rng = np.random.default_rng(2)
+    trajs = [rng.standard_normal((5000, 64)).astype(np.float32)]
+    a = _gpu(trajs, 5, _native.EMMA_PREC_BF16X4, reversible=True, remove_mean=True, c0t=True)
+    b = _gpu(trajs, 5, _native.EMMA_PREC_BF16X4, reversible=True, remove_mean=True, c0t=True)
+    for x, y in zip(a, b):
+        if x is not None:
+            np.testing.assert_array_equal(x, y)
