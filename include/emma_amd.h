@@ -51,6 +51,7 @@ extern "C" {
 /* arithmetic of the covariance tiles */
 #define EMMA_PREC_FP32 0    /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 tile accumulation */
 #define EMMA_PREC_BF16X4 1  /* operands split into bf16 hi+lo, 4 bf16 MFMA products, FP32 tile accumulation */
+#define EMMA_PREC_FP64 2    /* FP64 operands on v_mfma_f64_16x16x4_f64, FP64 accumulation */
 
 /* assignment modes */
 #define EMMA_ASSIGN_FP32_PLAIN 0    /* sequential FP32 loop per (frame, center) on the vector ALUs */

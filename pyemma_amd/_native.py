@@ -20,6 +20,7 @@ EMMA_MOM_C0T = 4
 EMMA_MOM_CTT = 8
 EMMA_PREC_FP32 = 0
 EMMA_PREC_BF16X4 = 1
+EMMA_PREC_FP64 = 2
 EMMA_ASSIGN_FP32_PLAIN = 0
 EMMA_ASSIGN_BF16X3_RECHECK = 1
 

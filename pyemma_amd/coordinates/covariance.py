@@ -17,7 +17,7 @@ from ._data import to_device
 
 log = logging.getLogger("pyemma_amd.coordinates.estimation.covariance.LaggedCovariance")
 
-PRECISIONS = {"bf16x4": _native.EMMA_PREC_BF16X4, "fp32": _native.EMMA_PREC_FP32}
+PRECISIONS = {"bf16x4": _native.EMMA_PREC_BF16X4, "fp32": _native.EMMA_PREC_FP32, "fp64": _native.EMMA_PREC_FP64}
 
 
 class Moments(object):
@@ -91,9 +91,10 @@ class LaggedCovariance(object):
     r"""Compute lagged covariances between time series (GPU moments).
 
     Parameters follow covariance.py:96-125; `precision` selects the
-    covariance-tile arithmetic ('bf16x4': bf16 hi/lo split on MFMA, 'fp32':
-    FP32 MFMA); both accumulate tiles in FP32 over at most 1024 pairs and
-    merge in FP64.
+    covariance-tile arithmetic: 'bf16x4' (bf16 hi/lo split on MFMA) and 'fp32'
+    (FP32 MFMA) accumulate tiles in FP32 over at most 1024 pairs and merge in
+    FP64; 'fp64' runs FP64 MFMA with FP64 accumulation (agrees with the FP64
+    reference computation to rounding, also for near-zero eigenvalues).
     """
 
     def __init__(self, c00=True, c0t=False, ctt=False, remove_constant_mean=None, remove_data_mean=False,

@@ -270,31 +270,32 @@ template <int KSTEPS>
 struct MfmaLds {
     static constexpr int DP = KSTEPS * 16;     // padded dims
     static constexpr int PITCH = DP + 8;       // uint16 per center row: 16-byte reads conflict-free
-    alignas(16) uint16_t a[2][2][32 * PITCH];  // [buffer][hi|lo][center row]
-    alignas(16) float cst[2][3][32];           // [buffer][nb|q|r][center]
+    alignas(16) uint16_t a[2][32 * PITCH];     // [hi|lo][center row]
+    alignas(16) float cst[3][32];              // [nb|q|r][center]
 };
 
 template <int KSTEPS, int NW>
-__global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
-                                                         CenterPack cp, int k, int k_pad, float CE,
-                                                         float gref, int32_t *__restrict__ labels,
-                                                         int64_t *__restrict__ list,
-                                                         int64_t *__restrict__ list_count,
-                                                         const int32_t *__restrict__ done) {
+__global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
+                                                            CenterPack cp, int k, int k_pad, float CE,
+                                                            float gref, int32_t *__restrict__ labels,
+                                                            int64_t *__restrict__ list,
+                                                            int64_t *__restrict__ list_count,
+                                                            const int32_t *__restrict__ done) {
     if (done && *done) return;
     using L = MfmaLds<KSTEPS>;
     constexpr int DP = L::DP;
     constexpr int NT = NW * 64;
-    constexpr int CHUNKS = 32 * DP / 8;                    // 16-byte chunks per (tile, hi|lo)
-    constexpr int PER_T = (2 * CHUNKS + NT - 1) / NT;      // prefetch uint4 per thread
+    constexpr int CHUNKS = 32 * DP / 8;  // 16-byte chunks per (tile, hi|lo)
     __shared__ L lds;
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t frame = ((int64_t)blockIdx.x * NW + wave) * 32 + (lane & 31);
     const bool fvalid = frame < n;
+    const int64_t frow = (fvalid ? frame : 0) * (int64_t)d;  // always a valid row
     const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
 
-    // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo ----
+    // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo (branch-free) ----
     bf16x8 bhi[KSTEPS], blo[KSTEPS];
     float na_part = 0.0f;
 #pragma unroll
@@ -303,8 +304,9 @@ __global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = s * 16 + kh + q;
-            float a = 0.0f;
-            if (fvalid && i < d) a = __fsub_rn(X[frame * d + i], cp.mu[i]);
+            const bool ok = fvalid && i < d;
+            const float xv = X[frow + (i < d ? i : d - 1)];
+            const float a = ok ? __fsub_rn(xv, cp.mu[i]) : 0.0f;
             na_part = __fadd_rn(na_part, __fmul_rn(a, a));
             split_bf16(a, h[q], l[q]);
         }
@@ -316,48 +318,33 @@ __global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict
     const float na = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
     const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
 
-    // ---- center tile staging ----
-    const int ntiles = k_pad / 32;
-    uint4 pre[PER_T];
-#define EMMA_FETCH(t_)                                                                            \
-    _Pragma("unroll") for (int u = 0; u < PER_T; ++u) {                                          \
-        const int c = tid + u * NT;                                                               \
-        if (c < 2 * CHUNKS) {                                                                     \
-            const int part = c / CHUNKS, cc = c - part * CHUNKS;                                  \
-            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);                            \
-            const uint16_t *src = (part == 0 ? cp.hi : cp.lo) + ((int64_t)((t_) * 32 + row)) * DP + col8 * 8; \
-            pre[u] = *reinterpret_cast<const uint4 *>(src);                                       \
-        }                                                                                         \
-    }
-#define EMMA_STORE(t_, buf_)                                                                      \
-    _Pragma("unroll") for (int u = 0; u < PER_T; ++u) {                                          \
-        const int c = tid + u * NT;                                                               \
-        if (c < 2 * CHUNKS) {                                                                     \
-            const int part = c / CHUNKS, cc = c - part * CHUNKS;                                  \
-            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);                            \
-            *reinterpret_cast<uint4 *>(&lds.a[(buf_)][part][row * L::PITCH + col8 * 8]) = pre[u]; \
-        }                                                                                         \
-    }                                                                                             \
-    if (tid < 96) {                                                                               \
-        const int which = tid >> 5, c = tid & 31;                                                 \
-        const float *srcv = which == 0 ? cp.nb : (which == 1 ? cp.q : cp.r);                      \
-        lds.cst[(buf_)][which][c] = srcv[(t_) * 32 + c];                                          \
-    }
-
     float tmin = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
     int imin = 0x7fffffff, ilb1 = 0x7fffffff;
+    const int ntiles = k_pad / 32;
+    const uint16_t *ah_base = &lds.a[0][(lane & 31) * L::PITCH + kh];
+    const uint16_t *al_base = &lds.a[1][(lane & 31) * L::PITCH + kh];
 
-    EMMA_FETCH(0)
-    EMMA_STORE(0, 0)
-    __syncthreads();
     for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) { EMMA_FETCH(t + 1) }
+        // ---- stage the 32-center tile (single buffer; co-resident workgroups overlap) ----
+        __syncthreads();
+#pragma unroll 4
+        for (int c = tid; c < 2 * CHUNKS; c += NT) {
+            const int part = c / CHUNKS, cc = c - part * CHUNKS;
+            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);
+            const uint16_t *src = (part == 0 ? cp.hi : cp.lo) + ((int64_t)(t * 32 + row)) * DP + col8 * 8;
+            *reinterpret_cast<uint4 *>(&lds.a[part][row * L::PITCH + col8 * 8]) =
+                *reinterpret_cast<const uint4 *>(src);
+        }
+        if (tid < 96) {
+            const int which = tid >> 5, c = tid & 31;
+            const float *srcv = which == 0 ? cp.nb : (which == 1 ? cp.q : cp.r);
+            lds.cst[which][c] = srcv[t * 32 + c];
+        }
+        __syncthreads();
+
         floatx16 acc;
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-        const uint16_t *ah_base = &lds.a[buf][0][(lane & 31) * L::PITCH + kh];
-        const uint16_t *al_base = &lds.a[buf][1][(lane & 31) * L::PITCH + kh];
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
             const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
@@ -370,9 +357,9 @@ __global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c0 = 8 * g + 4 * (lane >> 5);
-            const float4 nbv = *reinterpret_cast<const float4 *>(&lds.cst[buf][0][c0]);
-            const float4 qv = *reinterpret_cast<const float4 *>(&lds.cst[buf][1][c0]);
-            const float4 rv = *reinterpret_cast<const float4 *>(&lds.cst[buf][2][c0]);
+            const float4 nbv = *reinterpret_cast<const float4 *>(&lds.cst[0][c0]);
+            const float4 qv = *reinterpret_cast<const float4 *>(&lds.cst[1][c0]);
+            const float4 rv = *reinterpret_cast<const float4 *>(&lds.cst[2][c0]);
             const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
             const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
             const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
@@ -382,25 +369,17 @@ __global__ __launch_bounds__(NW * 64) void k_assign_mfma(const float *__restrict
                 const int j = t * 32 + c0 + h;
                 const float tv = fmaf(-2.0f, p, nba[h]);
                 const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
-                if (tv < tmin) {
-                    tmin = tv;
-                    imin = j;
-                }
-                if (lb < lb1) {
-                    lb2 = lb1;
-                    lb1 = lb;
-                    ilb1 = j;
-                } else if (lb < lb2) {
-                    lb2 = lb;
-                }
+                const bool c1 = tv < tmin;
+                tmin = c1 ? tv : tmin;
+                imin = c1 ? j : imin;
+                const bool c2 = lb < lb1, c3 = lb < lb2;
+                lb2 = c2 ? lb1 : (c3 ? lb : lb2);
+                ilb1 = c2 ? j : ilb1;
+                lb1 = c2 ? lb : lb1;
             }
         }
-        if (t + 1 < ntiles) { EMMA_STORE(t + 1, buf ^ 1) }
-        __syncthreads();
     }
 
-#undef EMMA_FETCH
-#undef EMMA_STORE
     // ---- merge the two half-waves that hold the same frame ----
     {
         const float ot = __shfl_xor(tmin, 32, 64);
@@ -464,7 +443,7 @@ static double gamma_ref(int d) {
 
 static int pick_ksteps(int d) {
     const int ks = (int)cdiv(d, 16);
-    static const int sizes[] = {2, 4, 8, 12, 16, 20, 24, 32};
+    static const int sizes[] = {2, 4, 8, 12, 16, 19, 24, 32};
     for (int s : sizes)
         if (ks <= s) return s;
     return -1;
@@ -539,7 +518,7 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
         case 8: launch_mfma_ks<8>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
         case 12: launch_mfma_ks<12>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
         case 16: launch_mfma_ks<16>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 20: launch_mfma_ks<20>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 19: launch_mfma_ks<19>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
         case 24: launch_mfma_ks<24>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
         case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
     }

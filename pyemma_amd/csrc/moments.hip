@@ -13,24 +13,35 @@
 //   raw - S S^T / w (Chan's formula collapsed to one global shift), the raw
 //   sums are S + w K.
 //
-// Parallel layout: the d x d outputs are cut into 128 x 128 tiles (tasks);
-// the pair index space is cut into slices.  One workgroup = (task, slice).
-// Per stage of BT pairs it stages the operand columns of its two column
-// blocks in LDS (bf16 hi/lo, or FP32), then 4 waves (2 x 2) each run a
-// 64 x 64 block of 32x32 MFMA tiles.  FP32 MFMA accumulators are flushed
-// every FLUSH pairs into the workgroup's private FP64 slot in the workspace,
-// so no FP32 chain is longer than FLUSH pairs.  A final kernel sums the
-// slots in slice order (deterministic) and applies the FP64 corrections.
+// Parallel layout: the d x d outputs are cut into TB x TB tiles (tasks); the
+// pair index space is cut into slices.  One workgroup = (task, slice).  Per
+// stage of BT = 16 pairs the workgroup stages the operand columns of its two
+// column blocks in LDS, then 4 waves (2 x 2) run MFMA tiles.  The frame loads
+// of stage s+1 are in flight while stage s runs on the matrix cores.
+//
+// Precisions:
+//   BF16X4  operands split into bf16 hi + lo, products hh, hl, lh, ll on
+//           v_mfma_f32_32x32x16_bf16, FP32 tile accumulation, TB = 128
+//   FP32    v_mfma_f32_32x32x2_f32, FP32 tile accumulation, TB = 128
+//   FP64    v_mfma_f64_16x16x4_f64 on FP64 operands (x - K exact in FP64),
+//           FP64 accumulation, TB = 64
+// FP32 accumulators are flushed every FLUSH pairs into the workgroup's private
+// FP64 slot in the workspace; a final kernel sums the slots in slice order
+// (deterministic) and applies the FP64 corrections.
+#include <type_traits>
+
 #include "emma_common.hpp"
 
 namespace emma {
 namespace mom {
 
-constexpr int TB = 128;      // output tile edge (columns per block)
-constexpr int NTHR = 256;    // threads per workgroup = 2 blocks x 128 columns
+constexpr int NTHR = 256;    // threads per workgroup
 constexpr int FLUSH = 1024;  // pairs per FP32 accumulation span
+constexpr int BT = 16;       // pairs per stage
 
 enum Kind { REV = 0, NR_SYM = 1, NR_XY = 2, LAG0 = 3 };
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 struct Params {
     const float *X;
@@ -39,26 +50,25 @@ struct Params {
     const int64_t *traj_off;
     int ntraj;
     int lag, stride, skip;
-    int nb;          // column blocks = ceil(d / TB)
-    int n_sym;       // number of I <= J tasks
+    int tb;               // tile edge
+    int frag;             // slot layout: 1 = MFMA fragment order (TB 128), 0 = row-major
+    int64_t slot_doubles; // doubles per slot: 2 matrices + 2 column-sum vectors
+    int nb;               // column blocks = ceil(d / tb)
+    int n_sym;            // number of I <= J tasks
     int n_tasks;
     int n_slices;
-    int rev;         // reversible
-    int ctt;         // compute Y'Y in NR_SYM tasks
-    int has_xy;      // non-reversible XY tasks present
+    int rev;              // reversible
+    int ctt;              // compute Y'Y in NR_SYM tasks
+    int has_xy;           // non-reversible XY tasks present
     // workspace
-    int64_t *P_total;      // [1]
-    int64_t *pair_off;     // [ntraj + 1]
-    float *K;              // [d]
-    double *slots;         // n_tasks * n_slices * SLOT_DOUBLES
+    int64_t *P_total;     // [1]
+    int64_t *pair_off;    // [ntraj + 1]
+    float *K;             // [d]
+    double *slots;        // n_tasks * n_slices * slot_doubles
 };
 
-constexpr int64_t SLOT_MAT = (int64_t)TB * TB;
-constexpr int64_t SLOT_DOUBLES = 2 * SLOT_MAT + 2 * TB;  // two matrices + two column-sum vectors
-
-__device__ __forceinline__ void task_of(const Params &p, int t, int &I, int &J, int &kind) {
+__device__ __forceinline__ void task_of(const Params &p, int t, int &I, int &J) {
     if (t < p.n_sym) {
-        // enumerate I <= J row by row
         int i = 0, rem = t;
         while (rem >= p.nb - i) {
             rem -= p.nb - i;
@@ -66,25 +76,19 @@ __device__ __forceinline__ void task_of(const Params &p, int t, int &I, int &J, 
         }
         I = i;
         J = i + rem;
-        kind = p.lag == 0 ? LAG0 : (p.rev ? REV : NR_SYM);
     } else {
-        int u = t - p.n_sym;
+        const int u = t - p.n_sym;
         I = u / p.nb;
         J = u - I * p.nb;
-        kind = NR_XY;
     }
-}
-
-__device__ __forceinline__ int nmat_of(const Params &p, int kind) {
-    return kind == REV ? 2 : (kind == NR_SYM && p.ctt ? 2 : 1);
 }
 
 // ------------------------------------------------------------------
 // prep: pair offsets (exclusive scan of per-trajectory pair counts), shift K
 // ------------------------------------------------------------------
 __device__ __forceinline__ int64_t traj_pairs(const Params &p, int t) {
-    int64_t T = p.traj_off[t + 1] - p.traj_off[t];
-    int64_t start = (int64_t)p.skip + p.lag;
+    const int64_t T = p.traj_off[t + 1] - p.traj_off[t];
+    const int64_t start = (int64_t)p.skip + p.lag;
     if (T <= start) return 0;
     return (T - start + p.stride - 1) / p.stride;
 }
@@ -96,13 +100,11 @@ __global__ __launch_bounds__(1024) void k_prep(Params p) {
     if (tid == 0) s_carry = 0;
     __syncthreads();
     for (int base = 0; base < p.ntraj; base += 1024) {
-        int t = base + tid;
-        int64_t v = t < p.ntraj ? traj_pairs(p, t) : 0;
-        s_part[tid] = v;
+        const int t = base + tid;
+        s_part[tid] = t < p.ntraj ? traj_pairs(p, t) : 0;
         __syncthreads();
-        // Hillis-Steele inclusive scan
-        for (int off = 1; off < 1024; off <<= 1) {
-            int64_t a = tid >= off ? s_part[tid - off] : 0;
+        for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+            const int64_t a = tid >= off ? s_part[tid - off] : 0;
             __syncthreads();
             s_part[tid] += a;
             __syncthreads();
@@ -119,14 +121,14 @@ __global__ __launch_bounds__(1024) void k_prep(Params p) {
     __syncthreads();
     // shift: average of up to 64 x-frames spread over the pair index space
     const int64_t P = s_carry;
-    int ns = P < 64 ? (int)P : 64;
+    const int ns = P < 64 ? (int)P : 64;
     for (int c = tid; c < p.d; c += 1024) {
         double acc = 0.0;
         int t = 0;
         for (int s = 0; s < ns; ++s) {
-            int64_t q = (int64_t)s * P / ns;
+            const int64_t q = (int64_t)s * P / ns;
             while (p.pair_off[t + 1] <= q) ++t;
-            int64_t row = p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride;
+            const int64_t row = p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride;
             acc += (double)p.X[row * p.d + c];
         }
         p.K[c] = ns > 0 ? (float)(acc / ns) : 0.0f;
@@ -137,32 +139,44 @@ __global__ __launch_bounds__(1024) void k_prep(Params p) {
 // main kernel
 // ------------------------------------------------------------------
 template <int PREC>
-struct Lds;
-
+struct Cfg;
 template <>
-struct Lds<EMMA_PREC_BF16X4> {
-    static constexpr int BT = 16;           // pairs per stage (one 32x32x16 K-step)
-    static constexpr int PITCH = BT + 8;    // uint16 per column (48 bytes: conflict-free 16B reads)
-    static constexpr int SLOT = TB * PITCH; // uint16 per (slot, hi|lo)
-    alignas(16) uint16_t v[4][2][SLOT];
+struct Cfg<EMMA_PREC_BF16X4> {
+    static constexpr int TB = 128;
+    static constexpr int PITCH = BT + 8;  // uint16 per column (48 B: conflict-free 16-byte reads)
+    typedef uint16_t elem;
+    static constexpr int PARTS = 2;       // hi, lo
 };
-
 template <>
-struct Lds<EMMA_PREC_FP32> {
-    static constexpr int BT = 16;
-    static constexpr int PITCH = BT + 1;    // odd pitch: conflict-free column reads
-    static constexpr int SLOT = TB * PITCH;
-    alignas(16) float v[4][SLOT];
+struct Cfg<EMMA_PREC_FP32> {
+    static constexpr int TB = 128;
+    static constexpr int PITCH = BT + 1;  // odd: conflict-free column reads
+    typedef float elem;
+    static constexpr int PARTS = 1;
+};
+template <>
+struct Cfg<EMMA_PREC_FP64> {
+    static constexpr int TB = 64;
+    static constexpr int PITCH = BT + 1;
+    typedef double elem;
+    static constexpr int PARTS = 1;
 };
 
 template <int PREC, int KIND, int NMAT>
 __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_launch_tasks) {
-    using L = Lds<PREC>;
-    constexpr int BT = L::BT;
+    using C = Cfg<PREC>;
+    constexpr int TB = C::TB;
+    constexpr int PITCH = C::PITCH;
+    constexpr int TPC = NTHR / (2 * TB);  // loader threads per column
+    constexpr int PPT = BT / TPC;         // pairs per loader thread
     constexpr bool SYM = KIND != NR_XY;
-    __shared__ L lds;
-    __shared__ int64_t s_xo[BT], s_yo[BT];  // element offsets of the pair's rows (-1: none)
-    __shared__ int s_cursor;
+    constexpr bool F64 = PREC == EMMA_PREC_FP64;
+    typedef typename C::elem elem;
+
+    __shared__ __align__(16) elem s_op[4][C::PARTS][TB * PITCH];  // slots A1, B1, A2, B2
+    __shared__ int64_t s_xo[2][BT];                               // element offset of the x row (-1: none)
+    __shared__ int s_tr[2][BT];                                   // trajectory of each pair
+    __shared__ double s_sum[2][TPC][TB];                          // column-sum partials (diagonal tasks)
 
     // XCD-aware numbering: workgroups are dealt round-robin to the 8 XCDs, so
     // give each XCD a contiguous range of (slice, task) pairs; the tasks of one
@@ -172,8 +186,8 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     if (wg >= n_launch_tasks * p.n_slices) return;
     const int task = task_base + wg % n_launch_tasks;
     const int slice = wg / n_launch_tasks;
-    int I, J, kind_unused;
-    task_of(p, task, I, J, kind_unused);
+    int I, J;
+    task_of(p, task, I, J);
     const bool diag = SYM && (I == J);
 
     const int64_t P = *p.P_total;
@@ -181,263 +195,397 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     const int64_t per_r = (per + BT - 1) / BT * BT;
     const int64_t p_lo = (int64_t)slice * per_r;
     const int64_t p_hi = p_lo + per_r < P ? p_lo + per_r : P;
-
-    double *slot = p.slots + ((int64_t)slice * p.n_tasks + task) * SLOT_DOUBLES;
+    double *slot = p.slots + ((int64_t)slice * p.n_tasks + task) * p.slot_doubles;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 1, wc = wave & 1;
 
-    // loader role: one column of block I (tid < 128) or block J
-    const int half = tid >> 7;
-    const int lcol = tid & 127;
+    // loader role: column lcol of block I (half 0) or J (half 1), pairs pq*PPT ..
+    const int half = tid / (TB * TPC);
+    const int lcol = (tid / TPC) % TB;
+    const int pq = tid % TPC;
     const int gcol_raw = (half == 0 ? I : J) * TB + lcol;
     const bool col_ok = gcol_raw < p.d;
     const int gcol = col_ok ? gcol_raw : p.d - 1;  // always a valid address
     const float Kc = p.K[gcol];
+    const int64_t lagd = (int64_t)p.lag * p.d;
     double sum_x = 0.0, sum_y = 0.0;
 
-    // subtile activity (wave-uniform)
-    bool act[2][2];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int rr = wr * 64 + r * 32, cc = wc * 64 + c * 32;
-            act[r][c] = (I * TB + rr < p.d) && (J * TB + cc < p.d) && !(diag && rr > cc);
+    // A wave computes all subtiles of its (TB/2 x TB/2) block or none: its block
+    // must intersect the valid d x d region and, on a diagonal tile, lie on or
+    // above the diagonal.  (Per-subtile predicates inside the stage loop make
+    // the compiler shuttle the accumulators between register files.)
+    constexpr int WT = TB / 2;  // wave tile edge
+    const bool wave_act = (I * TB + wr * WT < p.d) && (J * TB + wc * WT < p.d) && !(diag && wr > wc);
+
+    // ---- pair table: offsets of the x rows of one stage ----
+    auto build_table = [&](int b, int64_t s0, int t_start) {
+        if (tid < BT) {
+            const int64_t q = s0 + tid;
+            int t = t_start;
+            int64_t xo = -1;
+            if (q < p_hi) {
+                while (p.pair_off[t + 1] <= q) ++t;
+                xo = (p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride) * p.d;
+            }
+            s_xo[b][tid] = xo;
+            s_tr[b][tid] = t;
         }
-
-    floatx16 acc[NMAT][2][2];
-#pragma unroll
-    for (int m = 0; m < NMAT; ++m)
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[m][r][c][e] = 0.0f;
-
+    };
     if (tid == 0) {
-        // binary search: trajectory of the first pair of the slice
-        int lo = 0, hi = p.ntraj - 1;
+        int lo = 0, hi = p.ntraj - 1;  // trajectory of the slice's first pair
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (p.pair_off[mid] <= p_lo) lo = mid; else hi = mid - 1;
         }
-        s_cursor = lo;
+        s_tr[1][BT - 1] = lo;
     }
+    __syncthreads();
+    build_table(0, p_lo, s_tr[1][BT - 1]);
+    __syncthreads();
 
-    bool first_flush = true;
-    int64_t f0 = p_lo;
-    do {
-        const int64_t f1 = f0 + FLUSH < p_hi ? f0 + FLUSH : p_hi;
-        for (int64_t s0 = f0; s0 < f1; s0 += BT) {
-            __syncthreads();  // previous stage's LDS reads done; cursor visible
-            if (tid < BT) {
-                const int64_t q = s0 + tid;
-                int64_t xo = -1, yo = -1;
-                if (q < p_hi) {
-                    int t = s_cursor;
-                    while (p.pair_off[t + 1] <= q) ++t;
-                    const int64_t rx = p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride;
-                    xo = rx * p.d;
-                    yo = (rx + p.lag) * p.d;
-                }
-                s_xo[tid] = xo;
-                s_yo[tid] = yo;
-            }
-            __syncthreads();
-            if (tid == BT - 1 && s_xo[BT - 1] >= 0) {
-                int t = s_cursor;
-                const int64_t q = s0 + BT - 1;
-                while (p.pair_off[t + 1] <= q) ++t;
-                s_cursor = t;
-            }
-
-            // ---- load + form operands for this column, 8 pairs at a time ----
-            const int sl1 = half == 0 ? 0 : 1;  // slots: 0 = A1 (I), 1 = B1 (J), 2 = A2 (I), 3 = B2 (J)
-            const int sl2 = half == 0 ? 2 : 3;
-#pragma unroll 1
-            for (int g = 0; g < BT; g += 8) {
-                float o1[8], o2[8];
-                float bx = 0.0f, by = 0.0f;
+    // ---- prefetch registers ----
+    float xr[PPT], yr[PPT];
+    uint32_t vmask = 0;
+    auto issue_loads = [&](int b) {
+        vmask = 0;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int64_t xo = s_xo[g + q];
-                    const bool ok = (xo >= 0) && col_ok;
-                    const float xr = p.X[(xo >= 0 ? xo : 0) + gcol];
-                    float yr = 0.0f;
-                    if constexpr (KIND != LAG0) yr = p.X[(xo >= 0 ? s_yo[g + q] : 0) + gcol];
-                    const float x = ok ? xr : 0.0f;
-                    const float y = ok ? yr : 0.0f;
-                    const float xs = ok ? x - Kc : 0.0f;
-                    const float ys = ok ? y - Kc : 0.0f;
-                    bx += xs;
-                    by += ys;
-                    if constexpr (KIND == REV) {
-                        o1[q] = xs + ys;
-                        o2[q] = x - y;
-                    } else if constexpr (KIND == NR_XY) {
-                        o1[q] = half == 0 ? xs : ys;
-                        o2[q] = 0.0f;
-                    } else {  // NR_SYM, LAG0
-                        o1[q] = xs;
-                        o2[q] = ys;
-                    }
+        for (int q = 0; q < PPT; ++q) {
+            const int64_t xo = s_xo[b][pq * PPT + q];
+            const bool v = xo >= 0;
+            vmask |= (v ? 1u : 0u) << q;
+            xr[q] = p.X[(v ? xo : 0) + gcol];
+            if constexpr (KIND != LAG0) yr[q] = p.X[(v ? xo + lagd : 0) + gcol];
+            else yr[q] = 0.0f;
+        }
+    };
+    issue_loads(0);
+    build_table(1, p_lo + BT, s_tr[0][BT - 1]);
+
+    const int sl1 = half == 0 ? 0 : 1;  // slots: 0 = A1 (I), 1 = B1 (J), 2 = A2 (I), 3 = B2 (J)
+    const int sl2 = half == 0 ? 2 : 3;
+    int stage = 0;
+
+    // ---- operands of one stage from the prefetched values -> LDS ----
+    auto store_operands = [&]() {
+        float o1[PPT], o2[PPT];
+        double d1[PPT], d2[PPT];
+        float bx = 0.0f, by = 0.0f;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const bool ok = ((vmask >> q) & 1u) && col_ok;
+            const float x = ok ? xr[q] : 0.0f;
+            const float y = ok ? yr[q] : 0.0f;
+            if constexpr (F64) {
+                const double xs = ok ? (double)x - (double)Kc : 0.0;
+                const double ys = ok ? (double)y - (double)Kc : 0.0;
+                sum_x += xs;
+                sum_y += ys;
+                if constexpr (KIND == REV) {
+                    d1[q] = xs + ys;
+                    d2[q] = (double)x - (double)y;
+                } else if constexpr (KIND == NR_XY) {
+                    d1[q] = half == 0 ? xs : ys;
+                    d2[q] = 0.0;
+                } else {
+                    d1[q] = xs;
+                    d2[q] = ys;
                 }
-                if (diag && half == 0) {
-                    sum_x += (double)bx;
-                    sum_y += (double)by;
+            } else {
+                const float xs = ok ? x - Kc : 0.0f;
+                const float ys = ok ? y - Kc : 0.0f;
+                bx += xs;
+                by += ys;
+                if constexpr (KIND == REV) {
+                    o1[q] = xs + ys;
+                    o2[q] = x - y;
+                } else if constexpr (KIND == NR_XY) {
+                    o1[q] = half == 0 ? xs : ys;
+                    o2[q] = 0.0f;
+                } else {
+                    o1[q] = xs;
+                    o2[q] = ys;
                 }
-                if constexpr (PREC == EMMA_PREC_BF16X4) {
-                    uint32_t h[4], l[4];
+            }
+        }
+        if constexpr (!F64) {
+            sum_x += (double)bx;
+            sum_y += (double)by;
+        }
+        elem *dst1 = &s_op[sl1][0][lcol * PITCH + pq * PPT];
+        elem *dst2 = &s_op[sl2][0][lcol * PITCH + pq * PPT];
+        if constexpr (PREC == EMMA_PREC_BF16X4) {
+#pragma unroll
+            for (int g = 0; g < PPT; g += 8) {
+                uint32_t h[4], l[4];
+#pragma unroll
+                for (int q = 0; q < 8; q += 2) {
+                    uint32_t ha, la, hb, lb;
+                    split_bf16(o1[g + q], ha, la);
+                    split_bf16(o1[g + q + 1], hb, lb);
+                    h[q / 2] = pack2(ha, hb);
+                    l[q / 2] = pack2(la, lb);
+                }
+                *reinterpret_cast<uint4 *>(dst1 + g) = make_uint4(h[0], h[1], h[2], h[3]);
+                *reinterpret_cast<uint4 *>(dst1 + TB * PITCH + g) = make_uint4(l[0], l[1], l[2], l[3]);
+                if constexpr (NMAT == 2) {
 #pragma unroll
                     for (int q = 0; q < 8; q += 2) {
                         uint32_t ha, la, hb, lb;
-                        split_bf16(o1[q], ha, la);
-                        split_bf16(o1[q + 1], hb, lb);
+                        split_bf16(o2[g + q], ha, la);
+                        split_bf16(o2[g + q + 1], hb, lb);
                         h[q / 2] = pack2(ha, hb);
                         l[q / 2] = pack2(la, lb);
                     }
-                    *reinterpret_cast<uint4 *>(&lds.v[sl1][0][lcol * L::PITCH + g]) = make_uint4(h[0], h[1], h[2], h[3]);
-                    *reinterpret_cast<uint4 *>(&lds.v[sl1][1][lcol * L::PITCH + g]) = make_uint4(l[0], l[1], l[2], l[3]);
-                    if constexpr (NMAT == 2) {
-#pragma unroll
-                        for (int q = 0; q < 8; q += 2) {
-                            uint32_t ha, la, hb, lb;
-                            split_bf16(o2[q], ha, la);
-                            split_bf16(o2[q + 1], hb, lb);
-                            h[q / 2] = pack2(ha, hb);
-                            l[q / 2] = pack2(la, lb);
-                        }
-                        *reinterpret_cast<uint4 *>(&lds.v[sl2][0][lcol * L::PITCH + g]) = make_uint4(h[0], h[1], h[2], h[3]);
-                        *reinterpret_cast<uint4 *>(&lds.v[sl2][1][lcol * L::PITCH + g]) = make_uint4(l[0], l[1], l[2], l[3]);
-                    }
-                } else {
-                    float *d1 = &lds.v[sl1][lcol * L::PITCH + g];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) d1[q] = o1[q];
-                    if constexpr (NMAT == 2) {
-                        float *d2 = &lds.v[sl2][lcol * L::PITCH + g];
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) d2[q] = o2[q];
-                    }
+                    *reinterpret_cast<uint4 *>(dst2 + g) = make_uint4(h[0], h[1], h[2], h[3]);
+                    *reinterpret_cast<uint4 *>(dst2 + TB * PITCH + g) = make_uint4(l[0], l[1], l[2], l[3]);
                 }
             }
-            __syncthreads();
+        } else if constexpr (PREC == EMMA_PREC_FP32) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) dst1[q] = o1[q];
+            if constexpr (NMAT == 2) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) dst2[q] = o2[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) dst1[q] = d1[q];
+            if constexpr (NMAT == 2) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) dst2[q] = d2[q];
+            }
+        }
+    };
 
-            // ---- MFMA ----
+    // ---- MFMA of one stage; FIRST starts the accumulation from a zero C operand ----
+    floatx16 acc[NMAT][2][2];
+    doublex4 acc64[NMAT][2][2];
+    auto mfma_stage = [&](auto first_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;
 #pragma unroll
-            for (int m = 0; m < NMAT; ++m) {
-                const int sa = m == 0 ? 0 : 2, sb = m == 0 ? 1 : 3;
-                if constexpr (PREC == EMMA_PREC_BF16X4) {
-                    bf16x8 ah[2], al[2], bh[2], bl[2];
+        for (int m = 0; m < NMAT; ++m) {
+            const int sa = m == 0 ? 0 : 2, sb = m == 0 ? 1 : 3;
+            if constexpr (PREC == EMMA_PREC_BF16X4) {
+                bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        const int col = wr * 64 + r * 32 + (lane & 31);
-                        const int off = col * L::PITCH + (lane >> 5) * 8;
-                        ah[r] = *reinterpret_cast<const bf16x8 *>(&lds.v[sa][0][off]);
-                        al[r] = *reinterpret_cast<const bf16x8 *>(&lds.v[sa][1][off]);
-                    }
+                for (int r = 0; r < 2; ++r) {
+                    const int off = (wr * 64 + r * 32 + (lane & 31)) * PITCH + (lane >> 5) * 8;
+                    ah[r] = *reinterpret_cast<const bf16x8 *>(&s_op[sa][0][off]);
+                    al[r] = *reinterpret_cast<const bf16x8 *>(&s_op[sa][0][TB * PITCH + off]);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int off = (wc * 64 + c * 32 + (lane & 31)) * PITCH + (lane >> 5) * 8;
+                    bh[c] = *reinterpret_cast<const bf16x8 *>(&s_op[sb][0][off]);
+                    bl[c] = *reinterpret_cast<const bf16x8 *>(&s_op[sb][0][TB * PITCH + off]);
+                }
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
-                        const int col = wc * 64 + c * 32 + (lane & 31);
-                        const int off = col * L::PITCH + (lane >> 5) * 8;
-                        bh[c] = *reinterpret_cast<const bf16x8 *>(&lds.v[sb][0][off]);
-                        bl[c] = *reinterpret_cast<const bf16x8 *>(&lds.v[sb][1][off]);
+                        // small products first
+                        if constexpr (FIRST) {
+                            const floatx16 z = {};
+                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], z, 0, 0, 0);
+                        } else {
+                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], acc[m][r][c], 0, 0, 0);
+                        }
+                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[m][r][c], 0, 0, 0);
+                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[m][r][c], 0, 0, 0);
+                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[m][r][c], 0, 0, 0);
                     }
+            } else if constexpr (PREC == EMMA_PREC_FP32) {
+#pragma unroll
+                for (int kk = 0; kk < BT; kk += 2) {
+                    float av[2], bv[2];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+                        av[r] = s_op[sa][0][(wr * 64 + r * 32 + (lane & 31)) * PITCH + kk + (lane >> 5)];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        bv[c] = s_op[sb][0][(wc * 64 + c * 32 + (lane & 31)) * PITCH + kk + (lane >> 5)];
 #pragma unroll
                     for (int r = 0; r < 2; ++r)
 #pragma unroll
                         for (int c = 0; c < 2; ++c) {
-                            if (!act[r][c]) continue;
-                            // small products first
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], acc[m][r][c], 0, 0, 0);
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[m][r][c], 0, 0, 0);
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[m][r][c], 0, 0, 0);
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[m][r][c], 0, 0, 0);
-                        }
-                } else {
-#pragma unroll
-                    for (int kk = 0; kk < BT; kk += 2) {
-                        float av[2], bv[2];
-#pragma unroll
-                        for (int r = 0; r < 2; ++r) {
-                            const int col = wr * 64 + r * 32 + (lane & 31);
-                            av[r] = lds.v[sa][col * L::PITCH + kk + (lane >> 5)];
-                        }
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            const int col = wc * 64 + c * 32 + (lane & 31);
-                            bv[c] = lds.v[sb][col * L::PITCH + kk + (lane >> 5)];
-                        }
-#pragma unroll
-                        for (int r = 0; r < 2; ++r)
-#pragma unroll
-                            for (int c = 0; c < 2; ++c) {
-                                if (!act[r][c]) continue;
+                            if (FIRST && kk == 0) {
+                                const floatx16 z = {};
+                                acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r], bv[c], z, 0, 0, 0);
+                            } else {
                                 acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r], bv[c], acc[m][r][c], 0, 0, 0);
                             }
-                    }
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < BT; kk += 4) {
+                    double av[2], bv[2];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+                        av[r] = s_op[sa][0][(wr * 32 + r * 16 + (lane & 15)) * PITCH + kk + (lane >> 4)];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        bv[c] = s_op[sb][0][(wc * 32 + c * 16 + (lane & 15)) * PITCH + kk + (lane >> 4)];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            if (FIRST && kk == 0) {
+                                const doublex4 z = {};
+                                acc64[m][r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r], bv[c], z, 0, 0, 0);
+                            } else {
+                                acc64[m][r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r], bv[c], acc64[m][r][c], 0, 0, 0);
+                            }
+                        }
                 }
             }
         }
-        // ---- flush the FP32 span into the workgroup's FP64 slot ----
-        // Slot layout is the MFMA fragment layout (see frag_index): each lane owns
-        // 16 consecutive doubles per 32x32 tile, addressed from one base pointer.
-#pragma unroll
-        for (int m = 0; m < NMAT; ++m)
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    double *dst = slot + m * SLOT_MAT + ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16;
-#pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const double v = (double)acc[m][r][c][e];
-                        // the slot is private to this workgroup and every element has
-                        // one writer lane: the add needs no atomicity; the L2-side
-                        // FP64 add keeps the old value out of registers
-                        if (first_flush)
-                            dst[e] = v;
-                        else
-                            unsafeAtomicAdd(dst + e, v);
-                        acc[m][r][c][e] = 0.0f;
-                    }
-                }
-        first_flush = false;
-        f0 = f1;
-    } while (f0 < p_hi);
+    };
 
-    // column sums (diagonal tasks own them)
-    if (diag && half == 0) {
-        double *sums = slot + 2 * SLOT_MAT;
-        sums[lcol] = sum_x;
-        sums[TB + lcol] = sum_y;
+    // ---- one stage: operands -> LDS, loads of the next stage, MFMA, table ----
+    auto run_stage = [&](int64_t s0, auto mf_tag, auto first_tag) {
+        const int b = stage & 1;
+        store_operands();
+        __syncthreads();     // operands of stage s and the table of stage s+1 are visible
+        issue_loads(b ^ 1);  // stage s+1 in flight during the MFMAs below
+        if constexpr (decltype(mf_tag)::value) mfma_stage(first_tag);
+        build_table(b, s0 + 2 * BT, s_tr[b ^ 1][BT - 1]);  // stage s+2 into the buffer of stage s
+        __syncthreads();     // MFMA reads of the operands done before they are overwritten
+        ++stage;
+    };
+
+    // ---- flush into the workgroup's FP64 slot ----
+    auto flush = [&](bool first_flush) {
+        if constexpr (F64) {
+            // row-major slot, written once at the end
+#pragma unroll
+            for (int m = 0; m < NMAT; ++m)
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int j = wc * 32 + c * 16 + (lane & 15);
+                        double *dst = slot + (int64_t)m * TB * TB + (int64_t)(wr * 32 + r * 16 + 4 * (lane >> 4)) * TB + j;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) dst[(int64_t)e * TB] = acc64[m][r][c][e];
+                    }
+        } else {
+            // fragment-order slot: each lane owns 16 consecutive doubles per 32x32 tile
+#pragma unroll
+            for (int m = 0; m < NMAT; ++m)
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        double *dst = slot + (int64_t)m * TB * TB + ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16;
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+                            const double v = (double)acc[m][r][c][e];
+                            // the slot is private to this workgroup and every element has one
+                            // writer lane: the add needs no atomicity; the L2-side FP64 add
+                            // keeps the old value out of registers
+                            if (first_flush)
+                                dst[e] = v;
+                            else
+                                unsafeAtomicAdd(dst + e, v);
+                        }
+                    }
+        }
+    };
+
+    // ---- slice loop: spans of FLUSH pairs (FP64: one span) ----
+    typedef std::integral_constant<bool, true> T_;
+    typedef std::integral_constant<bool, false> F_;
+    if (wave_act) {
+        bool first_flush = true;
+        int64_t f0 = p_lo;
+        do {
+            const int64_t f1 = F64 ? p_hi : (f0 + FLUSH < p_hi ? f0 + FLUSH : p_hi);
+            run_stage(f0, T_(), T_());
+            for (int64_t s0 = f0 + BT; s0 < f1; s0 += BT) run_stage(s0, T_(), F_());
+            flush(first_flush);
+            first_flush = false;
+            f0 = f1;
+        } while (f0 < p_hi);
+    } else {
+        // same loads and barriers, no matrix work
+        int64_t f0 = p_lo;
+        do {
+            const int64_t f1 = F64 ? p_hi : (f0 + FLUSH < p_hi ? f0 + FLUSH : p_hi);
+            run_stage(f0, F_(), T_());
+            for (int64_t s0 = f0 + BT; s0 < f1; s0 += BT) run_stage(s0, F_(), F_());
+            f0 = f1;
+        } while (f0 < p_hi);
+    }
+
+    // column sums (diagonal tasks own them), combined over the loader threads of a column
+    if (diag) {
+        if (half == 0) {
+            s_sum[0][pq][lcol] = sum_x;
+            s_sum[1][pq][lcol] = sum_y;
+        }
+        __syncthreads();
+        if (tid < TB) {
+            double ax = 0.0, ay = 0.0;
+#pragma unroll
+            for (int u = 0; u < TPC; ++u) {
+                ax += s_sum[0][u][tid];
+                ay += s_sum[1][u][tid];
+            }
+            double *sums = slot + 2LL * TB * TB;
+            sums[tid] = ax;
+            sums[TB + tid] = ay;
+        }
     }
 }
 
 // ------------------------------------------------------------------
 // reduction + FP64 corrections
 // ------------------------------------------------------------------
+__device__ __forceinline__ int sym_task(const Params &p, int I, int J) {
+    int t = 0;
+    for (int i = 0; i < I; ++i) t += p.nb - i;
+    return t + (J - I);
+}
+
+// Position of tile element (li, lj) inside a slot matrix.  Fragment layout:
+// wave (li/64, lj/64) of the 2x2 wave grid, 32x32 subtile (r, c), and inside
+// it the 32x32 MFMA output layout lane = jj + 32*((ii>>2)&1), e = 4*(ii>>3) + (ii&3).
+__device__ __forceinline__ int64_t slot_index(const Params &p, int li, int lj) {
+    if (!p.frag) return (int64_t)li * p.tb + lj;
+    const int wave = (li >> 6) * 2 + (lj >> 6);
+    const int r = (li >> 5) & 1, c = (lj >> 5) & 1;
+    const int ii = li & 31, jj = lj & 31;
+    const int lane = jj + 32 * ((ii >> 2) & 1);
+    const int e = 4 * (ii >> 3) + (ii & 3);
+    return ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16 + e;
+}
+
+__device__ __forceinline__ double slot_sum(const Params &p, int task, int m, int li, int lj) {
+    const int64_t off = (int64_t)m * p.tb * p.tb + slot_index(p, li, lj);
+    double a = 0.0;
+    for (int s = 0; s < p.n_slices; ++s) a += p.slots[((int64_t)s * p.n_tasks + task) * p.slot_doubles + off];
+    return a;
+}
+
 // sums: S_x[c], S_y[c] (shifted) -> out sx/sy and workspace copies
 __global__ __launch_bounds__(256) void k_reduce_sums(Params p, double *Sx, double *Sy, double *w_out,
                                                      double *sx_out, double *sy_out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t P = *p.P_total;
-    const double w = p.lag == 0 ? (double)P : (p.rev ? 2.0 * (double)P : (double)P);
+    const double w = p.rev ? 2.0 * (double)P : (double)P;
     if (c == 0) *w_out = w;
     if (c >= p.d) return;
-    const int I = c / TB, lc = c - I * TB;
-    // diagonal task index of block I in the I <= J enumeration
-    int t = 0;
-    for (int i = 0; i < I; ++i) t += p.nb - i;
+    const int I = c / p.tb, lc = c - I * p.tb;
+    const int t = sym_task(p, I, I);
     double ax = 0.0, ay = 0.0;
     for (int s = 0; s < p.n_slices; ++s) {
-        const double *sums = p.slots + ((int64_t)s * p.n_tasks + t) * SLOT_DOUBLES + 2 * SLOT_MAT;
+        const double *sums = p.slots + ((int64_t)s * p.n_tasks + t) * p.slot_doubles + 2LL * p.tb * p.tb;
         ax += sums[lc];
-        ay += sums[TB + lc];
+        ay += sums[p.tb + lc];
     }
     const double K = (double)p.K[c];
     Sx[c] = ax;
@@ -455,46 +603,21 @@ __global__ __launch_bounds__(256) void k_reduce_sums(Params p, double *Sx, doubl
     }
 }
 
-// Position of tile element (li, lj) in the fragment layout written by k_main:
-// wave (li/64, lj/64) of the 2x2 wave grid, 32x32 subtile (r, c), and inside it
-// the 32x32 MFMA output layout: lane = jj + 32*((ii>>2)&1), e = 4*(ii>>3) + (ii&3).
-__device__ __forceinline__ int64_t frag_index(int li, int lj) {
-    const int wave = (li >> 6) * 2 + (lj >> 6);
-    const int r = (li >> 5) & 1, c = (lj >> 5) & 1;
-    const int ii = li & 31, jj = lj & 31;
-    const int lane = jj + 32 * ((ii >> 2) & 1);
-    const int e = 4 * (ii >> 3) + (ii & 3);
-    return ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16 + e;
-}
-
-__device__ __forceinline__ double slot_sum(const Params &p, int task, int m, int li, int lj) {
-    const int64_t off = m * SLOT_MAT + frag_index(li, lj);
-    double a = 0.0;
-    for (int s = 0; s < p.n_slices; ++s) a += p.slots[((int64_t)s * p.n_tasks + task) * SLOT_DOUBLES + off];
-    return a;
-}
-
-__device__ __forceinline__ int sym_task(const Params &p, int I, int J) {
-    int t = 0;
-    for (int i = 0; i < I; ++i) t += p.nb - i;
-    return t + (J - I);
-}
-
 // one thread per output element (i, j)
 __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx, const double *Sy,
                                                      int remove_mean, double *Mxx, double *Mxy,
                                                      double *Myy) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int d = p.d;
+    const int d = p.d, tb = p.tb;
     if (e >= (int64_t)d * d) return;
     const int i = (int)(e / d), j = (int)(e - (int64_t)i * d);
     const int64_t P = *p.P_total;
-    const int I = i / TB, J = j / TB, li = i - I * TB, lj = j - J * TB;
+    const int I = i / tb, J = j / tb, li = i - I * tb, lj = j - J * tb;
     // symmetric matrices are taken from the upper tile triangle (mirrored)
     const int is = i <= j ? i : j, js = i <= j ? j : i;
-    const int Is = is / TB, Js = js / TB;
+    const int Is = is / tb, Js = js / tb;
     const int ts = sym_task(p, Is, Js);
-    const int lis = is - Is * TB, ljs = js - Js * TB;
+    const int lis = is - Is * tb, ljs = js - Js * tb;
     const double K_i = (double)p.K[i], K_j = (double)p.K[j];
     if (P == 0) {
         if (Mxx) Mxx[e] = 0.0;
@@ -505,8 +628,7 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
     if (p.lag == 0) {
         const double raw = slot_sum(p, ts, 0, lis, ljs);
         const double w = (double)P;
-        Mxx[e] = remove_mean ? raw - Sx[i] * Sx[j] / w
-                             : raw + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
+        Mxx[e] = remove_mean ? raw - Sx[i] * Sx[j] / w : raw + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
         return;
     }
     if (p.rev) {
@@ -556,8 +678,10 @@ static void launch_kind(const Params &p, hipStream_t s) {
 static int launch_main(const Params &p, int precision, hipStream_t s) {
     if (precision == EMMA_PREC_BF16X4)
         launch_kind<EMMA_PREC_BF16X4>(p, s);
-    else
+    else if (precision == EMMA_PREC_FP32)
         launch_kind<EMMA_PREC_FP32>(p, s);
+    else
+        launch_kind<EMMA_PREC_FP64>(p, s);
     return check_launch("moments main");
 }
 
@@ -567,8 +691,8 @@ struct Plan {
     double *Sx, *Sy;
 };
 
-static int make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int64_t *traj_off, int ntraj,
-                     int lag, int stride, int skip, uint32_t flags, void *ws) {
+static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int64_t *traj_off, int ntraj,
+                      int lag, int stride, int skip, uint32_t flags, int precision, void *ws) {
     Params &p = pl.p;
     p.X = X;
     p.n_rows = n_rows;
@@ -578,15 +702,18 @@ static int make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int6
     p.lag = lag;
     p.stride = stride;
     p.skip = skip;
-    p.nb = (int)cdiv(d, TB);
+    p.tb = precision == EMMA_PREC_FP64 ? 64 : 128;
+    p.frag = precision == EMMA_PREC_FP64 ? 0 : 1;
+    p.slot_doubles = 2LL * p.tb * p.tb + 2LL * p.tb;
+    p.nb = (int)cdiv(d, p.tb);
     p.n_sym = p.nb * (p.nb + 1) / 2;
     p.rev = (flags & EMMA_MOM_SYMMETRIZE) && lag > 0 ? 1 : 0;
     p.ctt = (flags & EMMA_MOM_CTT) && lag > 0 && !p.rev ? 1 : 0;
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
     // about 3 workgroups per CU on 256 CUs, at least ~4k pairs per slice
-    int64_t want = cdiv(768, p.n_tasks);
-    int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 4096);
+    const int64_t want = cdiv(768, p.n_tasks);
+    const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 4096);
     p.n_slices = (int)(want < cap ? want : cap);
     if (p.n_slices < 1) p.n_slices = 1;
     Carver cv(ws);
@@ -595,9 +722,8 @@ static int make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int6
     p.K = cv.take<float>(d);
     pl.Sx = cv.take<double>(d);
     pl.Sy = cv.take<double>(d);
-    p.slots = cv.take<double>((size_t)p.n_tasks * p.n_slices * SLOT_DOUBLES);
+    p.slots = cv.take<double>((size_t)p.n_tasks * p.n_slices * p.slot_doubles);
     pl.ws_bytes = cv.bytes();
-    return EMMA_OK;
 }
 
 }  // namespace mom
@@ -607,13 +733,11 @@ using namespace emma;
 
 extern "C" size_t emma_moments_workspace_size(int64_t n_rows, int32_t d, int32_t ntraj, uint32_t flags,
                                               int32_t precision) {
-    (void)precision;
     if (d <= 0 || ntraj < 0) return 0;
-    mom::Plan pl;
-    // lag only matters through lag == 0; size for the larger (lagged) task set
-    mom::make_plan(pl, nullptr, n_rows, d, nullptr, ntraj, 1, 1, 0, flags, nullptr);
-    mom::Plan pl0;
-    mom::make_plan(pl0, nullptr, n_rows, d, nullptr, ntraj, 0, 1, 0, flags, nullptr);
+    mom::Plan pl, pl0;
+    // lag only matters through lag == 0; size for the larger task set
+    mom::make_plan(pl, nullptr, n_rows, d, nullptr, ntraj, 1, 1, 0, flags, precision, nullptr);
+    mom::make_plan(pl0, nullptr, n_rows, d, nullptr, ntraj, 0, 1, 0, flags, precision, nullptr);
     return pl.ws_bytes > pl0.ws_bytes ? pl.ws_bytes : pl0.ws_bytes;
 }
 
@@ -624,7 +748,7 @@ extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, co
                                    void *stream) {
     EMMA_REQUIRE(d > 0 && ntraj > 0 && n_rows >= 0, "emma_moments_lagged: bad sizes d=%d ntraj=%d", d, ntraj);
     EMMA_REQUIRE(lag >= 0 && stride >= 1 && skip >= 0, "emma_moments_lagged: bad lag/stride/skip");
-    EMMA_REQUIRE(precision == EMMA_PREC_FP32 || precision == EMMA_PREC_BF16X4,
+    EMMA_REQUIRE(precision == EMMA_PREC_FP32 || precision == EMMA_PREC_BF16X4 || precision == EMMA_PREC_FP64,
                  "emma_moments_lagged: unknown precision %d", precision);
     EMMA_REQUIRE(X && traj_offsets && w && sx && Mxx, "emma_moments_lagged: NULL pointer");
     const bool rev = (flags & EMMA_MOM_SYMMETRIZE) && lag > 0;
@@ -634,20 +758,19 @@ extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, co
     EMMA_REQUIRE(!(flags & EMMA_MOM_C0T) || Mxy, "emma_moments_lagged: Mxy is NULL");
     EMMA_REQUIRE(!(flags & EMMA_MOM_CTT) || Myy, "emma_moments_lagged: Myy is NULL");
     mom::Plan pl;
-    mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, nullptr);
+    mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, nullptr);
     EMMA_REQUIRE(workspace && workspace_bytes >= pl.ws_bytes,
                  "emma_moments_lagged: workspace too small (%zu < %zu)", workspace_bytes, pl.ws_bytes);
-    mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, workspace);
+    mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, workspace);
     hipStream_t s = (hipStream_t)stream;
     const mom::Params &p = pl.p;
     mom::k_prep<<<1, 1024, 0, s>>>(p);
     EMMA_TRY(check_launch("moments prep"));
-    EMMA_TRY(launch_main(p, precision, s));
+    EMMA_TRY(mom::launch_main(p, precision, s));
     mom::k_reduce_sums<<<(unsigned)cdiv(d, 256), 256, 0, s>>>(p, pl.Sx, pl.Sy, w, sx, sy);
     EMMA_TRY(check_launch("moments reduce sums"));
     const int rm = (flags & EMMA_MOM_REMOVE_MEAN) ? 1 : 0;
     mom::k_reduce_mats<<<(unsigned)cdiv((int64_t)d * d, 256), 256, 0, s>>>(
-        p, pl.Sx, pl.Sy, rm, Mxx, p.lag > 0 && (flags & EMMA_MOM_C0T) ? Mxy : nullptr,
-        p.ctt ? Myy : nullptr);
+        p, pl.Sx, pl.Sy, rm, Mxx, p.lag > 0 && (flags & EMMA_MOM_C0T) ? Mxy : nullptr, p.ctt ? Myy : nullptr);
     return check_launch("moments reduce");
 }
