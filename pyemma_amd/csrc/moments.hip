@@ -460,7 +460,8 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     // ---- flush into the workgroup's FP64 slot ----
     auto flush = [&](bool first_flush) {
         if constexpr (F64) {
-            // row-major slot, written once at the end
+            // row-major slot, written once at the end.  16x16x4 f64 output layout:
+            // lane l, element e holds row (l >> 4) + 4 e, column l & 15.
 #pragma unroll
             for (int m = 0; m < NMAT; ++m)
 #pragma unroll
@@ -468,9 +469,9 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
                         const int j = wc * 32 + c * 16 + (lane & 15);
-                        double *dst = slot + (int64_t)m * TB * TB + (int64_t)(wr * 32 + r * 16 + 4 * (lane >> 4)) * TB + j;
+                        double *dst = slot + (int64_t)m * TB * TB + (int64_t)(wr * 32 + r * 16 + (lane >> 4)) * TB + j;
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) dst[(int64_t)e * TB] = acc64[m][r][c][e];
+                        for (int e = 0; e < 4; ++e) dst[(int64_t)(4 * e) * TB] = acc64[m][r][c][e];
                     }
         } else {
             // fragment-order slot: each lane owns 16 consecutive doubles per 32x32 tile
