@@ -9,7 +9,7 @@
 //
 //  * k_assign_plain: the sequential FP32 loop itself on the vector ALUs
 //    (4 frames x 4 centers per thread, LDS-staged tiles).
-//  * k_assign_mfma + k_recheck: distances from bf16 hi/lo MFMA products on
+//  * k_assign_mfma, then k_assign_plain on a list: distances from bf16 hi/lo MFMA products on
 //    shifted data, d~_j = |a|^2 + |b_j|^2 - 2 a.b_j with a = fl(x - mu),
 //    b_j = fl(c_j - mu).  With E_j = CE (|a| + |b_j|)^2 a rigorous bound on
 //    |d~_j - d_j| (CE from u = 2^-24, the bf16 split residual 2^-16 and the
@@ -18,7 +18,7 @@
 //    decided when
 //        (1 - g) min_{j != best} (d~_j - E_j)  >  (1 + g)(d~_best + E_best);
 //    then no other center can reach the reference loop's minimum.  All other
-//    frames are appended to a list and re-evaluated by k_recheck with the
+//    frames are appended to a list and re-evaluated by k_assign_plain with the
 //    sequential FP32 loop over every center.
 #include <math.h>
 
@@ -59,9 +59,11 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
     const int tid = threadIdx.x;
     const int fg = tid >> 4;  // frames fg*4 .. +3
     const int cg = tid & 15;  // centers cg*4 .. +3 within the tile
-    const int64_t f0 = (int64_t)blockIdx.x * PF;
     const int64_t nrows = list ? *list_count : n;
-    if (f0 >= nrows) return;
+    // grid-stride over blocks of PF frames (a listed run sizes its grid for the
+    // whole device and reads the list length on the device)
+    for (int64_t f0 = (int64_t)blockIdx.x * PF; f0 < nrows; f0 += (int64_t)gridDim.x * PF) {
+    __syncthreads();  // s_row / s_best of the previous block are consumed
     if (tid < PF) {
         const int64_t f = f0 + tid;
         s_row[tid] = f < nrows ? (list ? list[f] : f) : -1;
@@ -139,46 +141,9 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
                 bi = vi;
             }
         }
-        if (s_row[tid] >= 0) labels[s_row[tid]] = bi;
+        // every distance +inf: the reference loop keeps center 0
+        if (s_row[tid] >= 0) labels[s_row[tid]] = bi == 0x7fffffff ? 0 : bi;
     }
-}
-
-// ------------------------------------------------------------------
-// exact recheck: one wavefront per listed frame, lanes split the centers
-// ------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_recheck(const float *__restrict__ X, int d, const float *__restrict__ C,
-                                                 int k, int32_t *__restrict__ labels,
-                                                 const int64_t *__restrict__ list,
-                                                 const int64_t *__restrict__ list_count,
-                                                 const int32_t *__restrict__ done) {
-    if (done && *done) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t cnt = *list_count;
-    const int64_t stride = (int64_t)gridDim.x * 4;
-    for (int64_t e = (int64_t)blockIdx.x * 4 + wave; e < cnt; e += stride) {
-        const int64_t row = list[e];
-        const float *x = X + row * d;  // every lane reads the same row: cache broadcast
-        float best = INFINITY;
-        int bi = 0x7fffffff;
-        for (int j = lane; j < k; j += 64) {
-            const float *c = C + (int64_t)j * d;
-            float acc = 0.0f;
-            for (int i = 0; i < d; ++i) acc = sq_step(acc, x[i], c[i]);
-            if (acc < best) {
-                best = acc;
-                bi = j;
-            }
-        }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const float ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov < best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) labels[row] = bi;
     }
 }
 
@@ -305,8 +270,13 @@ __global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma
         for (int q = 0; q < 8; ++q) {
             const int i = s * 16 + kh + q;
             const bool ok = fvalid && i < d;
+            // both loads unconditional (clamped row index; mu holds 16 KSTEPS
+            // entries) and the padding zeroed by a bit mask, so that no branch
+            // serializes the loads
             const float xv = X[frow + (i < d ? i : d - 1)];
-            const float a = ok ? __fsub_rn(xv, cp.mu[i]) : 0.0f;
+            const float mv = cp.mu[i];
+            const uint32_t keep = ok ? 0xffffffffu : 0u;
+            const float a = __uint_as_float(__float_as_uint(__fsub_rn(xv, mv)) & keep);
             na_part = __fadd_rn(na_part, __fmul_rn(a, a));
             split_bf16(a, h[q], l[q]);
         }
@@ -523,8 +493,10 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
         case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
     }
     EMMA_TRY(check_launch("assign mfma"));
-    // recheck: enough waves to cover the whole device; each loops over the list
-    k_recheck<<<1024, 256, 0, s>>>(X, d, C, k, labels, w.list, w.count, done);
+    // recheck: the exact loop over every center for the listed frames; the
+    // grid covers the device, the kernel reads the list length itself
+    const int64_t rb = cdiv(n, PF) < 2048 ? cdiv(n, PF) : 2048;
+    k_assign_plain<<<(unsigned)rb, 256, 0, s>>>(X, n, d, C, k, labels, w.list, w.count, done);
     EMMA_TRY(check_launch("assign recheck"));
     if (n_rechecked) k_copy_i64<<<1, 1, 0, s>>>(n_rechecked, w.count);
     return check_launch("assign copy count");

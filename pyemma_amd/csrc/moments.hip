@@ -221,7 +221,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     const bool wave_act = (I * TB + wr * WT < p.d) && (J * TB + wc * WT < p.d) && !(diag && wr > wc);
 
     // ---- pair table: offsets of the x rows of one stage ----
-    auto build_table = [&](int b, int64_t s0, int t_start) {
+    auto build_table = [&](int b, int64_t s0, int t_start) __attribute__((always_inline)) {
         if (tid < BT) {
             const int64_t q = s0 + tid;
             int t = t_start;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     // ---- prefetch registers ----
     float xr[PPT], yr[PPT];
     uint32_t vmask = 0;
-    auto issue_loads = [&](int b) {
+    auto issue_loads = [&](int b) __attribute__((always_inline)) {
         vmask = 0;
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     int stage = 0;
 
     // ---- operands of one stage from the prefetched values -> LDS ----
-    auto store_operands = [&]() {
+    auto store_operands = [&]() __attribute__((always_inline)) {
         float o1[PPT], o2[PPT];
         double d1[PPT], d2[PPT];
         float bx = 0.0f, by = 0.0f;
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     // ---- MFMA of one stage; FIRST starts the accumulation from a zero C operand ----
     floatx16 acc[NMAT][2][2];
     doublex4 acc64[NMAT][2][2];
-    auto mfma_stage = [&](auto first_tag) {
+    auto mfma_stage = [&](auto first_tag) __attribute__((always_inline)) {
         constexpr bool FIRST = decltype(first_tag)::value;
 #pragma unroll
         for (int m = 0; m < NMAT; ++m) {
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     };
 
     // ---- one stage: operands -> LDS, loads of the next stage, MFMA, table ----
-    auto run_stage = [&](int64_t s0, auto mf_tag, auto first_tag) {
+    auto run_stage = [&](int64_t s0, auto mf_tag, auto first_tag) __attribute__((always_inline)) {
         const int b = stage & 1;
         store_operands();
         __syncthreads();     // operands of stage s and the table of stage s+1 are visible
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     };
 
     // ---- flush into the workgroup's FP64 slot ----
-    auto flush = [&](bool first_flush) {
+    auto flush = [&](bool first_flush) __attribute__((always_inline)) {
         if constexpr (F64) {
             // row-major slot, written once at the end.  16x16x4 f64 output layout:
             // lane l, element e holds row (l >> 4) + 4 e, column l & 15.
