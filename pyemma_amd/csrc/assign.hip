@@ -152,27 +152,36 @@ __global__ void k_copy_i64(int64_t *dst, const int64_t *src) { *dst = *src; }
 // ------------------------------------------------------------------
 // center preparation for the MFMA path
 // ------------------------------------------------------------------
+// Centers are packed once per call into "tile images": for every tile of 32
+// centers one contiguous block that is byte-for-byte the LDS layout the MFMA
+// kernel reads,
+//   uint16 hi[32][pitch], lo[32][pitch]   bf16 bits of b = fl(c - mu), rows
+//                                          padded from d_pad to pitch = d_pad + 8
+//   float  nb[32], q[32], r[32]           |b|^2 (FP32 sequential), nb - CE bn^2, 2 CE bn
+// so that a tile moves from L2 to LDS with asynchronous 16-byte copies.
 struct CenterPack {
-    uint16_t *hi, *lo;  // [k_pad][d_pad] bf16 bits of b = fl(c - mu)
-    float *nb;          // |b|^2 (FP32 sequential); +inf for padding centers
-    float *q;           // nb - CE * bn^2
-    float *r;           // 2 * CE * bn
-    float *bn;          // |b| rounded up
-    float *mu;          // [d_pad] shift
+    uint8_t *img;  // k_pad / 32 tile images of img_bytes each
+    float *bn;     // [k_pad] |b| rounded up
+    float *mu;     // [d_pad] shift
 };
 
-// mu = mean of the centers (FP64, rounded to FP32)
+__host__ __device__ constexpr int tile_pitch(int d_pad) { return d_pad + 8; }
+__host__ __device__ constexpr int tile_img_bytes(int d_pad) { return 128 * tile_pitch(d_pad) + 3 * 32 * 4; }
+
+// mu = mean of the centers (FP64 in a fixed order, rounded to FP32):
+// 64 dims per workgroup, wave w sums centers w, w + 4, ...
 __global__ __launch_bounds__(256) void k_center_mean(const float *__restrict__ C, int k, int d, int d_pad,
                                                      float *__restrict__ mu) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d_pad) return;
-    if (i >= d) {
-        mu[i] = 0.0f;
-        return;
-    }
-    double s = 0.0;
-    for (int j = 0; j < k; ++j) s += (double)C[(int64_t)j * d + i];
-    mu[i] = (float)(s / k);
+    __shared__ double s_part[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (i < d)
+        for (int j = wave; j < k; j += 4) acc += (double)C[(int64_t)j * d + i];
+    s_part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && i < d_pad)
+        mu[i] = i < d ? (float)(((s_part[0][lane] + s_part[1][lane]) + (s_part[2][lane] + s_part[3][lane])) / k) : 0.0f;
 }
 
 // one wavefront per (padded) center
@@ -181,40 +190,46 @@ __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + wave;
     if (j >= k_pad) return;
+    const int pitch = tile_pitch(d_pad);
+    uint8_t *img = cp.img + (int64_t)(j >> 5) * tile_img_bytes(d_pad);
+    uint16_t *hi = reinterpret_cast<uint16_t *>(img) + (j & 31) * pitch;
+    uint16_t *lo = hi + 32 * pitch;
+    float *cst = reinterpret_cast<float *>(img + 128 * pitch);
     double n2 = 0.0;
-    for (int i0 = 0; i0 < d_pad; i0 += 64) {
-        const int i = i0 + lane;
+    for (int i = lane; i < pitch; i += 64) {
         float b = 0.0f;
         if (j < k && i < d) b = __fsub_rn(C[(int64_t)j * d + i], cp.mu[i]);
         uint32_t h, l;
         split_bf16(b, h, l);
-        if (i < d_pad) {
-            cp.hi[(int64_t)j * d_pad + i] = (uint16_t)h;
-            cp.lo[(int64_t)j * d_pad + i] = (uint16_t)l;
-        }
+        hi[i] = (uint16_t)h;  // the pad columns d_pad .. pitch - 1 hold zeros
+        lo[i] = (uint16_t)l;
         n2 += (double)b * (double)b;
     }
     // FP32 |b|^2 in index order (one lane walks the row; the error bound only
     // needs gamma_d, any order would do, this keeps it deterministic)
     double n2s = wave_butterfly_sum(n2);
     if (lane == 0) {
+        float nb, q, r, bn;
         if (j < k) {
             float acc = 0.0f;
             for (int i = 0; i < d; ++i) {
                 const float b = __fsub_rn(C[(int64_t)j * d + i], cp.mu[i]);
                 acc = __fadd_rn(acc, __fmul_rn(b, b));
             }
-            const float bn = (float)(sqrt(n2s) * (1.0 + 1e-6));  // rounded up
-            cp.nb[j] = acc;
-            cp.bn[j] = bn;
-            cp.q[j] = acc - CE * bn * bn;
-            cp.r[j] = 2.0f * CE * bn;
+            bn = (float)(sqrt(n2s) * (1.0 + 1e-6));  // rounded up
+            nb = acc;
+            q = acc - CE * bn * bn;
+            r = 2.0f * CE * bn;
         } else {
-            cp.nb[j] = INFINITY;
-            cp.bn[j] = 0.0f;
-            cp.q[j] = INFINITY;
-            cp.r[j] = 0.0f;
+            nb = INFINITY;
+            bn = 0.0f;
+            q = INFINITY;
+            r = 0.0f;
         }
+        cp.bn[j] = bn;
+        cst[j & 31] = nb;
+        cst[32 + (j & 31)] = q;
+        cst[64 + (j & 31)] = r;
     }
 }
 
@@ -227,31 +242,31 @@ __global__ void k_zero_i64n(int64_t *p, int n) {
 // MFMA kernel
 // ------------------------------------------------------------------
 // Wave w of a workgroup owns 32 frames; their bf16 hi/lo fragments stay in
-// registers for all centers.  Centers stream through LDS in tiles of 32,
-// double-buffered with a register prefetch.  v_mfma_f32_32x32x16_bf16:
-// A = 32 centers x 16 dims, B = 16 dims x 32 frames; lane l holds frame
-// l % 32 and centers 8(e>>2) + 4(l>>5) + (e&3), e = 0..15.
-template <int KSTEPS>
-struct MfmaLds {
-    static constexpr int DP = KSTEPS * 16;     // padded dims
-    static constexpr int PITCH = DP + 8;       // uint16 per center row: 16-byte reads conflict-free
-    alignas(16) uint16_t a[2][32 * PITCH];     // [hi|lo][center row]
-    alignas(16) float cst[3][32];              // [nb|q|r][center]
-};
+// registers for all centers.  Center tile images stream through two LDS
+// buffers: while the waves compute tile t from one buffer, the copy engine
+// path (global_load_lds, 16 bytes per lane) fills the other with tile t + 1.
+// v_mfma_f32_32x32x16_bf16: A = 32 centers x 16 dims, B = 16 dims x 32
+// frames; lane l holds frame l % 32 and centers 8(e>>2) + 4(l>>5) + (e&3).
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
 template <int KSTEPS, int NW>
-__global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
+__global__ __launch_bounds__(NW * 64, (KSTEPS <= 19 ? 2 : 1)) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float CE,
                                                             float gref, int32_t *__restrict__ labels,
                                                             int64_t *__restrict__ list,
                                                             int64_t *__restrict__ list_count,
                                                             const int32_t *__restrict__ done) {
     if (done && *done) return;
-    using L = MfmaLds<KSTEPS>;
-    constexpr int DP = L::DP;
+    constexpr int DP = KSTEPS * 16;
+    constexpr int PITCH = tile_pitch(DP);
+    constexpr int IMG = tile_img_bytes(DP);
+    constexpr int CHUNKS = IMG / 16;
     constexpr int NT = NW * 64;
-    constexpr int CHUNKS = 32 * DP / 8;  // 16-byte chunks per (tile, hi|lo)
-    __shared__ L lds;
+    constexpr int ROUNDS = (CHUNKS + NT - 1) / NT;
+    // two separate objects, so that the compiler can tell the buffer being
+    // read from the one being filled
+    __shared__ __align__(16) uint8_t s_img0[IMG];
+    __shared__ __align__(16) uint8_t s_img1[IMG];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -259,6 +274,19 @@ __global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma
     const bool fvalid = frame < n;
     const int64_t frow = (fvalid ? frame : 0) * (int64_t)d;  // always a valid row
     const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
+    const int ntiles = k_pad / 32;
+
+    // asynchronous copy of tile image t into an LDS buffer
+    auto fetch = [&](int t, uint8_t *dst) __attribute__((always_inline)) {
+        const uint8_t *src = cp.img + (int64_t)t * IMG;
+#pragma unroll
+        for (int i = 0; i < ROUNDS; ++i) {
+            const int c0 = i * NT + wave * 64;  // first chunk of this wave's 1 KB block
+            if (c0 + lane < CHUNKS)
+                __builtin_amdgcn_global_load_lds(src + (int64_t)(c0 + lane) * 16, (lds_ptr_t)(dst + c0 * 16), 16, 0, 0);
+        }
+    };
+    fetch(0, s_img0);
 
     // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo (branch-free) ----
     bf16x8 bhi[KSTEPS], blo[KSTEPS];
@@ -290,31 +318,12 @@ __global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma
 
     float tmin = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
     int imin = 0x7fffffff, ilb1 = 0x7fffffff;
-    const int ntiles = k_pad / 32;
-    const uint16_t *ah_base = &lds.a[0][(lane & 31) * L::PITCH + kh];
-    const uint16_t *al_base = &lds.a[1][(lane & 31) * L::PITCH + kh];
 
-    for (int t = 0; t < ntiles; ++t) {
-        // ---- stage the 32-center tile (single buffer; co-resident workgroups overlap) ----
-        __syncthreads();
-#pragma unroll 4
-        for (int c = tid; c < 2 * CHUNKS; c += NT) {
-            const int part = c / CHUNKS, cc = c - part * CHUNKS;
-            const int row = cc / (DP / 8), col8 = cc - row * (DP / 8);
-            const uint16_t *src = (part == 0 ? cp.hi : cp.lo) + ((int64_t)(t * 32 + row)) * DP + col8 * 8;
-            *reinterpret_cast<uint4 *>(&lds.a[part][row * L::PITCH + col8 * 8]) =
-                *reinterpret_cast<const uint4 *>(src);
-        }
-        if (tid < 96) {
-            const int which = tid >> 5, c = tid & 31;
-            const float *srcv = which == 0 ? cp.nb : (which == 1 ? cp.q : cp.r);
-            lds.cst[which][c] = srcv[t * 32 + c];
-        }
-        __syncthreads();
-
-        floatx16 acc;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    auto compute = [&](const uint8_t *img, int t) __attribute__((always_inline)) {
+        const uint16_t *ah_base = reinterpret_cast<const uint16_t *>(img) + (lane & 31) * PITCH + kh;
+        const uint16_t *al_base = ah_base + 32 * PITCH;
+        const float *cst = reinterpret_cast<const float *>(img + 128 * PITCH);
+        floatx16 acc = {};
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
             const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
@@ -327,9 +336,9 @@ __global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c0 = 8 * g + 4 * (lane >> 5);
-            const float4 nbv = *reinterpret_cast<const float4 *>(&lds.cst[0][c0]);
-            const float4 qv = *reinterpret_cast<const float4 *>(&lds.cst[1][c0]);
-            const float4 rv = *reinterpret_cast<const float4 *>(&lds.cst[2][c0]);
+            const float4 nbv = *reinterpret_cast<const float4 *>(&cst[c0]);
+            const float4 qv = *reinterpret_cast<const float4 *>(&cst[32 + c0]);
+            const float4 rv = *reinterpret_cast<const float4 *>(&cst[64 + c0]);
             const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
             const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
             const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
@@ -347,6 +356,23 @@ __global__ __launch_bounds__(NW * 64, (KSTEPS <= 20 ? 2 : 1)) void k_assign_mfma
                 ilb1 = c2 ? j : ilb1;
                 lb1 = c2 ? lb : lb1;
             }
+        }
+    };
+
+    // tiles in pairs: buffer 0 holds even tiles, buffer 1 odd tiles.  Each
+    // barrier follows a wait for this wave's copies, so the tile is complete
+    // for every wave, and every wave has finished reading the buffer that the
+    // next copy overwrites.
+    for (int t = 0; t < ntiles; t += 2) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        if (t + 1 < ntiles) fetch(t + 1, s_img1);
+        compute(s_img0, t);
+        if (t + 1 < ntiles) {
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            __syncthreads();
+            if (t + 2 < ntiles) fetch(t + 2, s_img0);
+            compute(s_img1, t + 1);
         }
     }
 
@@ -435,11 +461,7 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
         const int ks = pick_ksteps(d);
         const int d_pad = (ks > 0 ? ks : 1) * 16;
         const int k_pad = (int)cdiv(k, 32) * 32;
-        w.cp.hi = cv.take<uint16_t>((size_t)k_pad * d_pad);
-        w.cp.lo = cv.take<uint16_t>((size_t)k_pad * d_pad);
-        w.cp.nb = cv.take<float>(k_pad);
-        w.cp.q = cv.take<float>(k_pad);
-        w.cp.r = cv.take<float>(k_pad);
+        w.cp.img = cv.take<uint8_t>((size_t)(k_pad / 32) * tile_img_bytes(d_pad));
         w.cp.bn = cv.take<float>(k_pad);
         w.cp.mu = cv.take<float>(d_pad);
     }
@@ -479,7 +501,7 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
     const float CE = (float)assign_error_coeff(d, d_pad);
     const float gref = (float)gamma_ref(d);
     k_zero_i64n<<<1, 64, 0, s>>>(w.count, 1);
-    k_center_mean<<<(unsigned)cdiv(d_pad, 256), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
+    k_center_mean<<<(unsigned)cdiv(d_pad, 64), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
     k_center_pack<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, CE, w.cp);
     EMMA_TRY(check_launch("assign center pack"));
     switch (ks) {

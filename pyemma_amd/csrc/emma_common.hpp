@@ -73,6 +73,19 @@ __device__ __forceinline__ void split_bf16(float a, uint32_t &hi, uint32_t &lo) 
 
 __device__ __forceinline__ uint32_t pack2(uint32_t lo16, uint32_t hi16) { return lo16 | (hi16 << 16); }
 
+// Two values at once on gfx950's v_cvt_pk_bf16_f32 (round to nearest even):
+// hi = packed bf16(a0), bf16(a1); lo = packed bf16(a0 - hi0), bf16(a1 - hi1).
+// Same hi/lo as split_bf16 for finite inputs, in 6 instructions per pair.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_bf16x2(float a0, float a1, uint32_t &hi, uint32_t &lo) {
+    const f32x2_t v = {a0, a1};
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+    const f32x2_t hf = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
+    const f32x2_t r = v - hf;
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
+}
+
 // 64-lane butterfly sum (every lane ends with the same value).
 __device__ __forceinline__ double wave_butterfly_sum(double v) {
 #pragma unroll

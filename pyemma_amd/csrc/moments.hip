@@ -544,6 +544,336 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 }
 
 // ------------------------------------------------------------------
+// reversible moments, bf16 hi/lo split: the pipelined TICA kernel
+// ------------------------------------------------------------------
+// One workgroup = (task, slice) as above, 8 waves.  Per stage of RBT = 16
+// pairs the loader threads read x and y of their columns, form U = x' + y',
+// V = x - y, split both into bf16 hi + lo and write them to one of two LDS
+// buffers while the waves run the MFMAs of the previous stage from the other
+// buffer; one barrier per stage.  The raw frame values of a stage are loaded
+// two stages ahead into one of two register sets.  Wave roles:
+//   off-diagonal task (I < J): wave w computes matrix w >> 2 (UU | VV) on the
+//     64 x 64 block ((w >> 1) & 1, w & 1); loader thread t owns column t & 127
+//     of block I (group (t >> 7) even) or J (odd), pairs 8 ((t >> 7) >> 1) ..
+//   diagonal task (I == J): only block I is staged and serves as both MFMA
+//     operands; wave w computes matrix w & 1 on block (w >> 2, (w >> 1) & 1),
+//     the block below the diagonal is skipped; loader thread t owns column
+//     t & 127, pairs 4 (t >> 7) ..
+// The rows of a pair come from a per-wave cursor over the trajectory table
+// (the pairs of a wave are the same for all its lanes and increase
+// monotonically), so no pair table is staged.
+constexpr int RNT = 512;             // threads per workgroup
+constexpr int RTB = 128;             // tile edge
+constexpr int RBT = 16;              // pairs per stage
+constexpr int RPITCH = RBT + 8;      // uint16 per column row (48 B: conflict-free 16-byte reads)
+constexpr int RFLUSH = 2048;         // pairs per FP32 accumulation span
+constexpr int RSPAN = RFLUSH / RBT;  // stages per span (even)
+
+struct RevLds {
+    uint16_t op[2][4][2][RTB * RPITCH];  // [buffer][U_I, U_J, V_I, V_J][hi, lo][column row]
+};
+
+// scalar (wave-uniform) cursor: pair index -> x row
+struct PairCursor {
+    int t;
+    int64_t p0, p1, base;
+};
+
+template <bool DIAG>
+__global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
+    constexpr int PPT = DIAG ? 4 : 8;  // pairs per loader thread per stage
+    constexpr int TB = RTB;
+    __shared__ __align__(16) RevLds lds;
+    __shared__ double s_sum[4][TB];
+
+    // task of this launch and XCD-aware numbering as in k_main
+    const int n_launch_tasks = DIAG ? p.nb : p.n_sym - p.nb;
+    const int per_xcd = gridDim.x >> 3;
+    const int wg = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (wg >= n_launch_tasks * p.n_slices) return;
+    const int lt = wg % n_launch_tasks;
+    const int slice = wg / n_launch_tasks;
+    int I, J;
+    if (DIAG) {
+        I = J = lt;
+    } else {
+        int i = 0, rem = lt;  // lt-th pair I < J in row-major order
+        while (rem >= p.nb - 1 - i) {
+            rem -= p.nb - 1 - i;
+            ++i;
+        }
+        I = i;
+        J = i + 1 + rem;
+    }
+    int task = 0;
+    for (int i = 0; i < I; ++i) task += p.nb - i;
+    task += J - I;
+
+    const int64_t P = *p.P_total;
+    const int64_t per = (P + p.n_slices - 1) / p.n_slices;
+    const int64_t per_r = (per + RBT - 1) / RBT * RBT;
+    const int64_t p_lo = (int64_t)slice * per_r < P ? (int64_t)slice * per_r : P;
+    const int64_t p_hi = p_lo + per_r < P ? p_lo + per_r : P;
+    const int n_stages = (int)((p_hi - p_lo + RBT - 1) / RBT);
+    double *slot = p.slots + ((int64_t)slice * p.n_tasks + task) * p.slot_doubles;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    // ---- loader role ----
+    const int grp = tid >> 7;  // wave-uniform
+    const int lcol = tid & (TB - 1);
+    const int blk = DIAG ? I : ((grp & 1) ? J : I);
+    // first pair of this thread in a stage (wave-uniform: the cursor lives in scalar registers)
+    const int pfirst = __builtin_amdgcn_readfirstlane(DIAG ? grp * PPT : (grp >> 1) * PPT);
+    const int su = DIAG ? 0 : (grp & 1);                     // LDS slot of U (V at su + 2)
+    const int gcol_raw = blk * TB + lcol;
+    const bool col_ok = gcol_raw < p.d;
+    const int gcol = col_ok ? gcol_raw : p.d - 1;
+    const uint32_t col_keep = col_ok ? 1u : 0u;
+    const float Kc = p.K[gcol];
+    const int64_t lagd = (int64_t)p.lag * p.d;
+
+    // ---- MFMA role ----
+    int m, wr, wc;
+    if (DIAG) {
+        m = wave & 1;
+        wr = wave >> 2;
+        wc = (wave >> 1) & 1;
+    } else {
+        m = wave >> 2;
+        wr = (wave >> 1) & 1;
+        wc = wave & 1;
+    }
+    const bool wave_act = (I * TB + wr * 64 < p.d) && (J * TB + wc * 64 < p.d) && !(DIAG && wr > wc);
+    const int sa = m == 0 ? 0 : 2;
+    const int sb = DIAG ? sa : sa + 1;
+
+    // ---- cursor: trajectory of the slice's first pair ----
+    // (the tables are read-only here: uniform reads go through the scalar cache)
+    typedef const __attribute__((address_space(4))) int64_t *cptr;
+    const cptr poff_c = (cptr)p.pair_off;
+    const cptr toff_c = (cptr)p.traj_off;
+    PairCursor cur;
+    {
+        int lo = 0, hi = p.ntraj - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.pair_off[mid] <= p_lo) lo = mid; else hi = mid - 1;
+        }
+        cur.t = lo;
+        cur.p0 = p.pair_off[lo];
+        cur.p1 = p.pair_off[lo + 1];
+        cur.base = p.traj_off[lo] + p.skip;
+    }
+    // element offset of the x row of pair q (wave-uniform q; -1 if q is past the slice)
+    auto row_of = [&](int64_t q) __attribute__((always_inline)) -> int64_t {
+        if (q >= p_hi) return -1;
+        while (q >= cur.p1) {
+            ++cur.t;
+            cur.p0 = cur.p1;
+            cur.p1 = poff_c[cur.t + 1];
+            cur.base = toff_c[cur.t] + p.skip;
+        }
+        return (cur.base + (q - cur.p0) * p.stride) * p.d;
+    };
+
+    // element offsets of the x rows of one stage's pairs (scalar registers,
+    // computed one block ahead of the loads), the raw x, y of a stage (zero
+    // for padding pairs and columns) and the validity bit of each pair
+    int64_t ra[PPT], rb[PPT];
+    const int64_t step_d = (int64_t)p.stride * p.d;
+    auto rows_for = [&](int64_t *r, int s) __attribute__((always_inline)) {
+        const int64_t q0 = p_lo + (int64_t)s * RBT + pfirst;
+        if (q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi) {
+            // common case: all pairs in the cursor's trajectory
+            const int64_t r0 = (cur.base + (q0 - cur.p0) * p.stride) * p.d;
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) r[q] = r0 + q * step_d;
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) r[q] = row_of(q0 + q);
+        }
+    };
+    float xa[PPT], ya[PPT], xb[PPT], yb[PPT];
+    uint32_t va = 0, vb = 0;
+    // row pointers are wave-uniform (scalar), the column a 32-bit lane offset:
+    // no vector address arithmetic; values are masked when they are used
+    const uint32_t gcol_u = (uint32_t)gcol;
+    auto issue = [&](float *xr, float *yr, uint32_t &vm, const int64_t *rows) __attribute__((always_inline)) {
+        vm = 0;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int64_t xo = rows[q];
+            const float *xp = p.X + (xo >= 0 ? xo : 0);
+            const float *yp = xo >= 0 ? xp + lagd : p.X;
+            xr[q] = xp[gcol_u];
+            yr[q] = yp[gcol_u];
+            vm |= (xo >= 0 ? (col_keep & 1u) : 0u) << q;
+        }
+    };
+    double sum_u = 0.0;
+    auto prep = [&](const float *xr, const float *yr, uint32_t vm, int buf) __attribute__((always_inline)) {
+        uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
+        float bu = 0.0f;
+#pragma unroll
+        for (int q = 0; q < PPT; q += 2) {
+            // padding pairs and columns give U = V = 0
+            const bool ok0 = (vm >> q) & 1u, ok1 = (vm >> (q + 1)) & 1u;
+            const float u0 = ok0 ? __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc)) : 0.0f;
+            const float u1 = ok1 ? __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc)) : 0.0f;
+            const float v0 = ok0 ? __fsub_rn(xr[q], yr[q]) : 0.0f;
+            const float v1 = ok1 ? __fsub_rn(xr[q + 1], yr[q + 1]) : 0.0f;
+            bu = __fadd_rn(bu, __fadd_rn(u0, u1));
+            split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
+            split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
+        }
+        sum_u += (double)bu;
+        const int off = lcol * RPITCH + pfirst;
+        if constexpr (PPT == 8) {
+            *reinterpret_cast<uint4 *>(&lds.op[buf][su][0][off]) = make_uint4(uh[0], uh[1], uh[2], uh[3]);
+            *reinterpret_cast<uint4 *>(&lds.op[buf][su][1][off]) = make_uint4(ul[0], ul[1], ul[2], ul[3]);
+            *reinterpret_cast<uint4 *>(&lds.op[buf][su + 2][0][off]) = make_uint4(vh[0], vh[1], vh[2], vh[3]);
+            *reinterpret_cast<uint4 *>(&lds.op[buf][su + 2][1][off]) = make_uint4(vl[0], vl[1], vl[2], vl[3]);
+        } else {
+            *reinterpret_cast<uint2 *>(&lds.op[buf][su][0][off]) = make_uint2(uh[0], uh[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][su][1][off]) = make_uint2(ul[0], ul[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][su + 2][0][off]) = make_uint2(vh[0], vh[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][su + 2][1][off]) = make_uint2(vl[0], vl[1]);
+        }
+    };
+
+    floatx16 acc[2][2];
+    auto zero_acc = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = floatx16{};
+    };
+    auto mfma = [&](int buf) __attribute__((always_inline)) {
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int off = (wr * 64 + r * 32 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
+            ah[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][0][off]);
+            al[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][1][off]);
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int off = (wc * 64 + c * 32 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
+            bh[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][0][off]);
+            bl[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][1][off]);
+        }
+        // small products first
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+    };
+    // issue order inside one stage block: the operand reads, then each MFMA
+    // followed by a share of the next stage's split arithmetic and loads, so
+    // that the vector ALU work runs while the matrix core is busy
+    auto interleave = [&]() __attribute__((always_inline)) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS operand reads
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);            // 1 MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, DIAG ? 3 : 6, 0);  // VALU
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);            // 1 global load
+        }
+    };
+    auto flush = [&](bool first_flush) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double *dst = slot + (int64_t)m * TB * TB + ((int64_t)(((wr * 2 + wc) * 4 + r * 2 + c) * 64 + lane)) * 16;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const double v = (double)acc[r][c][e];
+                    // private slot, one writer lane per element: no atomicity needed;
+                    // the L2-side add keeps the old value out of registers
+                    if (first_flush)
+                        dst[e] = v;
+                    else
+                        unsafeAtomicAdd(dst + e, v);
+                }
+            }
+    };
+
+    // ---- pipeline: stage s runs its MFMAs from buffer s & 1 while stage s + 1
+    // is prepared into the other buffer from register set (s + 1) & 1, which is
+    // then refilled with stage s + 3 ----
+    const int n_even = n_stages > 0 ? (n_stages + 1) & ~1 : 2;  // at least one (zero) span: every slot is written
+    rows_for(ra, 0);
+    issue(xa, ya, va, ra);
+    rows_for(rb, 1);
+    issue(xb, yb, vb, rb);
+    prep(xa, ya, va, 0);
+    rows_for(ra, 2);
+    issue(xa, ya, va, ra);
+    zero_acc();
+    __syncthreads();
+    bool first_flush = true;
+    // waves without matrix work run the same loads, splits and barriers
+    auto run = [&](auto act_tag) __attribute__((always_inline)) {
+        constexpr bool ACT = decltype(act_tag)::value;
+        for (int s = 0; s < n_even; s += 2) {
+            // even stage s: MFMA buffer 0, prepare stage s + 1 (set b) into buffer 1
+            rows_for(rb, s + 3);
+            if constexpr (ACT) mfma(0);
+            prep(xb, yb, vb, 1);
+            issue(xb, yb, vb, rb);
+            if constexpr (ACT) interleave();
+            __syncthreads();
+            // odd stage s + 1: MFMA buffer 1, prepare stage s + 2 (set a) into buffer 0
+            rows_for(ra, s + 4);
+            if constexpr (ACT) mfma(1);
+            prep(xa, ya, va, 0);
+            issue(xa, ya, va, ra);
+            if constexpr (ACT) interleave();
+            __syncthreads();
+            if (((s + 2) % RSPAN) == 0 || s + 2 >= n_even) {
+                if constexpr (ACT) {
+                    flush(first_flush);
+                    zero_acc();
+                }
+                first_flush = false;
+            }
+        }
+    };
+    if (wave_act)
+        run(std::integral_constant<bool, true>());
+    else
+        run(std::integral_constant<bool, false>());
+
+    // ---- column sums of U (= shifted x + shifted y), diagonal tasks only ----
+    if (DIAG) {
+        s_sum[grp][lcol] = sum_u;
+        __syncthreads();
+        if (tid < TB) {
+            const double a = s_sum[0][tid] + s_sum[1][tid] + s_sum[2][tid] + s_sum[3][tid];
+            double *sums = slot + 2LL * TB * TB;
+            sums[tid] = a;
+            sums[TB + tid] = 0.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------
 // reduction + FP64 corrections
 // ------------------------------------------------------------------
 __device__ __forceinline__ int sym_task(const Params &p, int I, int J) {
@@ -661,6 +991,17 @@ template <int PREC>
 static void launch_kind(const Params &p, hipStream_t s) {
     const int n_sym = p.n_sym;
     const unsigned g_sym = (unsigned)(cdiv((int64_t)n_sym * p.n_slices, 8) * 8);
+    if constexpr (PREC == EMMA_PREC_BF16X4) {
+        if (p.rev) {
+            const unsigned g_diag = (unsigned)(cdiv((int64_t)p.nb * p.n_slices, 8) * 8);
+            k_rev_bf16<true><<<g_diag, RNT, 0, s>>>(p);
+            if (n_sym > p.nb) {
+                const unsigned g_off = (unsigned)(cdiv((int64_t)(n_sym - p.nb) * p.n_slices, 8) * 8);
+                k_rev_bf16<false><<<g_off, RNT, 0, s>>>(p);
+            }
+            return;
+        }
+    }
     if (p.lag == 0)
         k_main<PREC, LAG0, 1><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
     else if (p.rev)
@@ -712,10 +1053,17 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.ctt = (flags & EMMA_MOM_CTT) && lag > 0 && !p.rev ? 1 : 0;
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
-    // about 3 workgroups per CU on 256 CUs, at least ~4k pairs per slice
-    const int64_t want = cdiv(768, p.n_tasks);
-    const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 4096);
-    p.n_slices = (int)(want < cap ? want : cap);
+    if (precision == EMMA_PREC_BF16X4 && p.rev) {
+        // one 8-wave workgroup per CU: 256 slices make every launch of
+        // k_rev_bf16 a whole number of rounds over the 256 CUs
+        const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
+        p.n_slices = (int)(cap < 256 ? cap : 256);
+    } else {
+        // about 3 workgroups per CU on 256 CUs, at least ~4k pairs per slice
+        const int64_t want = cdiv(768, p.n_tasks);
+        const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 4096);
+        p.n_slices = (int)(want < cap ? want : cap);
+    }
     if (p.n_slices < 1) p.n_slices = 1;
     Carver cv(ws);
     p.P_total = cv.take<int64_t>(1);
