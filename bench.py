@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 GPU comparison")
     ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
     ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-pool", type=int, default=1000000, help="frames per d in the pinned host data pool")
     return ap.parse_args()
 
 
@@ -149,6 +152,12 @@ def main():
     if not args.no_cpu:
         cpu, parity = cpu_baseline(args, X, offsets, C, labels, l)
 
+    serving = None
+    if args.serve_seconds > 0:
+        del X, sub, labels
+        torch.cuda.empty_cache()
+        serving = serving_leg(args)
+
     out = {
         "metric": METRIC, "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -158,8 +167,79 @@ def main():
                                "assign k=%d" % (args.ntraj, args.nframes, d, args.lag, args.k),
                    "frames": N, "lagged_pairs": P},
         "roofline": roofline, "kernels": kern, "plain_gpu": plain, "cpu_baseline": cpu, "parity": parity,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def serving_leg(args):
+    """Poisson arrivals of discretization requests against 8 resident models.
+
+    Models: d cycles over {64, 128, 300, 512}, k over {256, 1000, 4096},
+    projected dimension m = min(d, 32); request lengths log-uniform in
+    [1k, min(1M, pool)] frames, data windows of a pinned host pool per d
+    (Gaussian-mixture frames generated on the device).  Latency is the time
+    from the scheduled arrival to the host seeing the labels.
+    """
+    import torch
+
+    from pyemma_amd.serving import DiscretizationServer, ResidentModel, latency_summary
+    from pyemma_amd.synthetic import gaussian_mixture
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rng = np.random.default_rng(5)
+    dims, ks = [64, 128, 300, 512], [256, 1000, 4096]
+    pool_n = int(args.serve_pool)
+    pools, models = {}, {}
+    t0 = time.time()
+    for i, d in enumerate(dims):
+        Xd = gaussian_mixture(pool_n, d, seed=50 + i, device=dev)
+        host = torch.empty((pool_n, d), dtype=torch.float32, pin_memory=True)
+        host.copy_(Xd)
+        pools[d] = (host, Xd[: min(pool_n, 20000)].double().mean(0).cpu().numpy())
+        del Xd
+    torch.cuda.synchronize()
+    for mi in range(8):
+        d, k = dims[mi % 4], ks[mi % 3]
+        m = min(d, 32)
+        host, mean = pools[d]
+        V, _ = np.linalg.qr(rng.standard_normal((d, m)))
+        rows = rng.choice(pool_n, size=k, replace=False)
+        C = ((host[torch.as_tensor(np.sort(rows))].double().numpy() - mean) @ V).astype(np.float32)
+        models[mi] = (d, ResidentModel(mean, V, C))
+    log("serving: pools and 8 models ready in %.1f s" % (time.time() - t0))
+    budget = 48 << 30
+    server = DiscretizationServer(budget, n_streams=8)
+    for key, (_, mdl) in models.items():
+        server.add_model(key, mdl)
+    # warm-up: one small request per model
+    for key, (d, _) in models.items():
+        server.submit(key, pools[d][0][:4096])
+    server.run_until_idle()
+    server.done.clear()
+    n_req = max(1, int(rng.poisson(args.serve_rate * args.serve_seconds)))
+    arrivals = np.sort(rng.uniform(0.0, args.serve_seconds, size=n_req))
+    lens = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(min(1e6, pool_n)), size=n_req))).astype(np.int64)
+    sched = []
+    for a, T in zip(arrivals, lens):
+        key = int(rng.integers(8))
+        d = models[key][0]
+        start = int(rng.integers(0, pool_n - T + 1))
+        sched.append((float(a), key, pools[d][0][start:start + T]))
+    t1 = time.perf_counter()
+    reqs = server.replay(sched)
+    wall = time.perf_counter() - t1
+    out = latency_summary(reqs)
+    out.update({"offered_req_per_s": args.serve_rate, "arrival_window_s": args.serve_seconds, "wall_s": wall,
+                "max_in_flight": server.max_running, "hbm_budget_gb": budget / 2 ** 30,
+                "peak_arena_gb": server.alloc.peak / 2 ** 30,
+                "offered_gb_per_s": float(sum(r.T * models[r.model_key][0] * 4 for r in reqs)) / 1e9
+                                    / args.serve_seconds,
+                "models": "8 resident: d in {64,128,300,512}, k in {256,1000,4096}, m = min(d, 32)",
+                "requests_lengths": "log-uniform [1e3, %d] frames" % min(10 ** 6, pool_n)})
+    log("serving: %d requests, p50 %.2f ms, p99 %.2f ms, %.3g frames/s, max in flight %d"
+        % (out["requests"], out["p50_ms"], out["p99_ms"], out["frames_per_s"], server.max_running))
+    return out
 
 
 def cpu_baseline(args, X, offsets, C, labels_gpu, l_gpu):

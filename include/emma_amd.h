@@ -126,6 +126,19 @@ int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k,
                 int32_t mode, int64_t *n_rechecked, void *workspace, size_t workspace_bytes,
                 void *stream);
 
+/*
+ * Tile variants of the MFMA assignment kernel (waves per workgroup and
+ * workgroups per CU), for the autotuner.  emma_assign_variants(d) is the
+ * number of variants at dimension d (0 if d is beyond the MFMA path);
+ * emma_assign_select(d, k, v) makes variant v the one used for every later
+ * call whose dimension and center count fall in the same bucket as (d, k)
+ * (process-wide, thread-safe; v = -1 restores the default).  Variants only
+ * change speed: labels are identical for every variant.
+ * No reference counterpart: the reference has no tunable kernels.
+ */
+int32_t emma_assign_variants(int32_t d);
+int emma_assign_select(int32_t d, int32_t k, int32_t variant);
+
 /* ------------------------------------------------------------------ */
 /* k-means                                                             */
 /* ------------------------------------------------------------------ */

@@ -52,45 +52,50 @@ def _f64(X):
     return np.array(X, dtype=np.float64, order="C", copy=True)
 
 
-def moments_XX(X, remove_mean=False):
-    """moments.py:601-696 restricted to the dense, unweighted path."""
+def moments_XX(X, remove_mean=False, weights=None):
+    """moments.py:601-696 restricted to the dense path (weights: moments.py:267-303, 540-553)."""
     X0 = _f64(X)
     T = X0.shape[0]
-    w = float(T)
-    sx = X0.sum(axis=0)                       # moments.py:281
+    wt = None if weights is None else np.asarray(weights, dtype=np.float64).ravel()[:T]
+    w = float(T) if wt is None else float(wt.sum())
+    sx = X0.sum(axis=0) if wt is None else (wt[:, None] * X0).sum(axis=0)   # moments.py:270-281
     if remove_mean:
         X0 -= sx / w                          # moments.py:326-358
-    Cxx = X0.T.dot(X0)                        # moments.py:386-401
+    Cxx = X0.T.dot(X0) if wt is None else (wt[:, None] * X0).T.dot(X0)    # moments.py:545-553
     return w, sx, Cxx
 
 
-def moments_XXXY(X, Y, remove_mean=False, symmetrize=False):
-    """moments.py:699-839 restricted to the dense, unweighted path.
+def moments_XXXY(X, Y, remove_mean=False, symmetrize=False, weights=None):
+    """moments.py:699-839 restricted to the dense path.
 
     symmetrize: w = 2T, sx = sy = sum X + sum Y (moments.py:290-296),
     Cxx = X0'X0 + Y0'Y0, Cxy = X0'Y0 + Y0'X0 (moments.py:548-559).
     Otherwise w = T, Cxx = X0'X0, Cxy = X0'Y0 with separately centered X, Y.
+    weights (one per pair, the weight of its x frame) multiply every term
+    and T becomes their sum (moments.py:267-303, 540-553).
     """
     X0 = _f64(X)
     Y0 = _f64(Y)
     T = X0.shape[0]
-    sx_raw = X0.sum(axis=0)
-    sy_raw = Y0.sum(axis=0)
+    wt = np.ones(T) if weights is None else np.asarray(weights, dtype=np.float64).ravel()[:T]
+    sx_raw = (wt[:, None] * X0).sum(axis=0)
+    sy_raw = (wt[:, None] * Y0).sum(axis=0)
     if symmetrize:
         sx = sx_raw + sy_raw
         sy = sx
-        w = 2.0 * T
+        w = 2.0 * wt.sum()
     else:
-        sx, sy, w = sx_raw, sy_raw, float(T)
+        sx, sy, w = sx_raw, sy_raw, float(wt.sum())
     if remove_mean:
         X0 -= sx / w
         Y0 -= sy / w
+    WX, WY = wt[:, None] * X0, wt[:, None] * Y0
     if symmetrize:
-        Cxx = X0.T.dot(X0) + Y0.T.dot(Y0)
-        Cxy = X0.T.dot(Y0) + Y0.T.dot(X0)
+        Cxx = WX.T.dot(X0) + WY.T.dot(Y0)
+        Cxy = WX.T.dot(Y0) + WY.T.dot(X0)
     else:
-        Cxx = X0.T.dot(X0)
-        Cxy = X0.T.dot(Y0)
+        Cxx = WX.T.dot(X0)
+        Cxy = WX.T.dot(Y0)
     return w, sx, sy, Cxx, Cxy
 
 
@@ -173,7 +178,7 @@ class MomentsStorage(object):
 
 def covariance_lagged(trajs, lag, c00=True, c0t=True, ctt=False, remove_data_mean=False,
                       reversible=False, bessel=True, stride=1, skip=0, chunksize=None,
-                      ncov_max=float("inf")):
+                      ncov_max=float("inf"), weights=None):
     """FP64 lagged covariances of a list of trajectories.
 
     Restates LaggedCovariance._estimate / _init_covar (covariance.py:160-252)
@@ -188,7 +193,7 @@ def covariance_lagged(trajs, lag, c00=True, c0t=True, ctt=False, remove_data_mea
     if (c0t or ctt) and lag == 0:
         raise ValueError("lag must be positive if c0t=True or ctt=True")
     chunks = []
-    for traj in trajs:
+    for i, traj in enumerate(trajs):
         traj = np.asarray(traj)
         if traj.ndim == 1:
             traj = traj[:, None]
@@ -200,27 +205,33 @@ def covariance_lagged(trajs, lag, c00=True, c0t=True, ctt=False, remove_data_mea
         n = len(X)
         if n == 0:
             continue
+        # frame weights follow the frames (covariance.py:198-227): the weight
+        # of a pair is the weight of its x frame
+        W = None if weights is None else np.asarray(weights[i], dtype=np.float64).ravel()[skip::stride][:n]
         cs = n if not chunksize else int(chunksize)
         for lo in range(0, n, cs):
-            chunks.append((X[lo:lo + cs], None if Y is None else Y[lo:lo + cs]))
+            chunks.append((X[lo:lo + cs], None if Y is None else Y[lo:lo + cs],
+                           None if W is None else W[lo:lo + cs]))
     if not chunks:
         raise ValueError("None single dataset is longer than lag+skip [%i]." % (lag + skip))
     nsave = min(int(max(math.log(len(chunks), 2), 2)), ncov_max)
     sxx = MomentsStorage(nsave, remove_mean=remove_data_mean)
     sxy = MomentsStorage(nsave, remove_mean=remove_data_mean) if c0t else None
     syy = MomentsStorage(nsave, remove_mean=remove_data_mean) if ctt else None
-    for X, Y in chunks:
+    for X, Y, W in chunks:
         if Y is None or (c00 and not c0t and not ctt):
-            w, sx, Cxx = moments_XX(X, remove_mean=remove_data_mean)
+            w, sx, Cxx = moments_XX(X, remove_mean=remove_data_mean, weights=W)
             sxx.store(Moments(w, sx, sx, Cxx))
         elif not ctt:
             w, sx, sy, Cxx, Cxy = moments_XXXY(X, Y, remove_mean=remove_data_mean,
-                                               symmetrize=reversible)
+                                               symmetrize=reversible, weights=W)
             sxx.store(Moments(w, sx, sx, Cxx))
             sxy.store(Moments(w, sx, sy, Cxy))
         else:
             if reversible:
                 raise ValueError("Combining compute_YY and symmetrize=True is meaningless.")
+            if W is not None:
+                raise NotImplementedError("weighted Ctt is not restated")
             w, s, C = moments_block(X, Y, remove_mean=remove_data_mean)
             sxx.store(Moments(w, s[0], s[0], C[0][0]))
             if c0t:

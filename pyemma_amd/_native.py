@@ -54,6 +54,10 @@ def lib():
     L.emma_assign_workspace_size.argtypes = [i64, i32, i32, i32]
     L.emma_assign.restype = ctypes.c_int
     L.emma_assign.argtypes = [vp, i64, i32, vp, i32, vp, i32, vp, vp, sz, vp]
+    L.emma_assign_variants.restype = i32
+    L.emma_assign_variants.argtypes = [i32]
+    L.emma_assign_select.restype = ctypes.c_int
+    L.emma_assign_select.argtypes = [i32, i32, i32]
     L.emma_kmeanspp_workspace_size.restype = sz
     L.emma_kmeanspp_workspace_size.argtypes = [i64, i32, i32]
     L.emma_kmeanspp.restype = ctypes.c_int

@@ -22,6 +22,8 @@
 //    sequential FP32 loop over every center.
 #include <math.h>
 
+#include <mutex>
+
 #include "emma_common.hpp"
 
 namespace emma {
@@ -249,8 +251,8 @@ __global__ void k_zero_i64n(int64_t *p, int n) {
 // frames; lane l holds frame l % 32 and centers 8(e>>2) + 4(l>>5) + (e&3).
 typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
-template <int KSTEPS, int NW>
-__global__ __launch_bounds__(NW * 64, (KSTEPS <= 19 ? 2 : 1)) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
+template <int KSTEPS, int NW, int MINB>
+__global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float CE,
                                                             float gref, int32_t *__restrict__ labels,
                                                             int64_t *__restrict__ list,
@@ -469,14 +471,72 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     return w;
 }
 
+// Tile variants (waves per workgroup, workgroups per CU) of the MFMA kernel
+// for each K-step count: every variant fits the registers and the two LDS
+// tile buffers of its occupancy.  Variant 0 is the default; the autotuner
+// (emma_assign_select) may pick another per (K steps, k) bucket.
+struct Variant {
+    int nw, minb;
+};
+static const Variant kVarSmall[] = {{4, 2}, {8, 1}, {8, 2}, {16, 1}};  // KSTEPS <= 8
+static const Variant kVarMid[] = {{4, 2}, {8, 1}};                    // 12, 16, 19
+static const Variant kVarLarge[] = {{4, 1}};                          // 24, 32
+
+static int n_variants(int ks) {
+    return ks <= 8 ? (int)(sizeof(kVarSmall) / sizeof(Variant))
+                   : (ks <= 19 ? (int)(sizeof(kVarMid) / sizeof(Variant)) : (int)(sizeof(kVarLarge) / sizeof(Variant)));
+}
+
+// process-wide selection table: [K-step class][ceil(log2(k_pad / 32))]
+static std::mutex g_sel_mu;
+static int g_sel[8][16];
+static bool g_sel_init = false;
+
+static int ks_index(int ks) {
+    static const int sizes[] = {2, 4, 8, 12, 16, 19, 24, 32};
+    for (int i = 0; i < 8; ++i)
+        if (sizes[i] == ks) return i;
+    return -1;
+}
+static int k_bucket(int k_pad) {
+    int b = 0;
+    while ((32 << b) < k_pad && b < 15) ++b;
+    return b;
+}
+static int selected_variant(int ks, int k_pad) {
+    std::lock_guard<std::mutex> lk(g_sel_mu);
+    if (!g_sel_init) return 0;
+    const int v = g_sel[ks_index(ks)][k_bucket(k_pad)];
+    return v < n_variants(ks) ? v : 0;
+}
+
+template <int KS, int NW, int MINB>
+static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE, float gref,
+                     int32_t *labels, const int32_t *done, hipStream_t s) {
+    const int64_t blocks = cdiv(n, 32 * NW);
+    k_assign_mfma<KS, NW, MINB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
+                                                                     w.list, w.count, done);
+}
+
 template <int KS>
 static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
-    // 4 waves per workgroup: 1-2 waves per SIMD given the fragment registers
-    constexpr int NW = 4;
-    const int64_t blocks = cdiv(n, 32 * NW);
-    k_assign_mfma<KS, NW><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels, w.list,
-                                                               w.count, done);
+    const int v = selected_variant(KS, k_pad);
+    if constexpr (KS <= 8) {
+        switch (v) {
+            case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 2: launch_v<KS, 8, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 3: launch_v<KS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+        }
+    } else if constexpr (KS <= 19) {
+        if (v == 1)
+            launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+        else
+            launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+    } else {
+        launch_v<KS, 4, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+    }
 }
 
 }  // namespace asg
@@ -545,4 +605,25 @@ extern "C" int emma_assign(const float *X, int64_t n, int32_t d, const float *C,
     EMMA_REQUIRE(workspace && workspace_bytes >= need, "emma_assign: workspace too small (%zu < %zu)",
                  workspace_bytes, need);
     return assign_run(X, n, d, C, k, labels, mode, n_rechecked, workspace, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int32_t emma_assign_variants(int32_t d) {
+    const int ks = asg::pick_ksteps(d);
+    return ks > 0 ? asg::n_variants(ks) : 0;
+}
+
+extern "C" int emma_assign_select(int32_t d, int32_t k, int32_t variant) {
+    using namespace asg;
+    EMMA_REQUIRE(d > 0 && k > 0, "emma_assign_select: bad sizes d=%d k=%d", d, k);
+    const int ks = pick_ksteps(d);
+    EMMA_REQUIRE(ks > 0, "emma_assign_select: d=%d exceeds the MFMA path limit (512)", d);
+    EMMA_REQUIRE(variant < n_variants(ks), "emma_assign_select: variant %d of %d", variant, n_variants(ks));
+    std::lock_guard<std::mutex> lk(g_sel_mu);
+    if (!g_sel_init) {
+        for (auto &row : g_sel)
+            for (int &v : row) v = 0;
+        g_sel_init = true;
+    }
+    g_sel[ks_index(ks)][k_bucket((int)cdiv(k, 32) * 32)] = variant < 0 ? 0 : variant;
+    return EMMA_OK;
 }

@@ -566,7 +566,7 @@ constexpr int RNT = 512;             // threads per workgroup
 constexpr int RTB = 128;             // tile edge
 constexpr int RBT = 16;              // pairs per stage
 constexpr int RPITCH = RBT + 8;      // uint16 per column row (48 B: conflict-free 16-byte reads)
-constexpr int RFLUSH = 2048;         // pairs per FP32 accumulation span
+constexpr int RFLUSH = 1024;         // pairs per FP32 accumulation span
 constexpr int RSPAN = RFLUSH / RBT;  // stages per span (even)
 
 struct RevLds {

@@ -95,3 +95,16 @@ def test_api_fails_loudly_without_gpu():
     from pyemma_amd.coordinates import tica
     with pytest.raises(_native.EmmaError):
         tica(np.random.rand(100, 3).astype(np.float32), lag=2)
+
+
+def test_assign_variant_table():
+    """Tile-variant queries and selection are host-only calls."""
+    L = _native.lib()
+    assert L.emma_assign_variants(32) == 4
+    assert L.emma_assign_variants(300) == 2
+    assert L.emma_assign_variants(512) == 1
+    assert L.emma_assign_variants(513) == 0
+    assert L.emma_assign_select(300, 1000, 5) == -1
+    assert L.emma_assign_select(600, 1000, 0) == -1
+    assert L.emma_assign_select(300, 1000, 1) == 0
+    assert L.emma_assign_select(300, 1000, -1) == 0
