@@ -474,14 +474,15 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
                         for (int e = 0; e < 4; ++e) dst[(int64_t)(4 * e) * TB] = acc64[m][r][c][e];
                     }
         } else {
-            // fragment-order slot: each lane owns 16 consecutive doubles per 32x32 tile
+            // fragment-order slot: element e of the 32x32 tile's lanes is 64
+            // consecutive doubles, so one instruction covers whole cache lines
 #pragma unroll
             for (int m = 0; m < NMAT; ++m)
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
-                        double *dst = slot + (int64_t)m * TB * TB + ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16;
+                        double *dst = slot + (int64_t)m * TB * TB + (int64_t)(wave * 4 + r * 2 + c) * 1024 + lane;
 #pragma unroll
                         for (int e = 0; e < 16; ++e) {
                             const double v = (double)acc[m][r][c][e];
@@ -489,9 +490,9 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
                             // writer lane: the add needs no atomicity; the L2-side FP64 add
                             // keeps the old value out of registers
                             if (first_flush)
-                                dst[e] = v;
+                                dst[e * 64] = v;
                             else
-                                unsafeAtomicAdd(dst + e, v);
+                                unsafeAtomicAdd(dst + e * 64, v);
                         }
                     }
         }
@@ -629,9 +630,9 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     const int pfirst = __builtin_amdgcn_readfirstlane(DIAG ? grp * PPT : (grp >> 1) * PPT);
     const int su = DIAG ? 0 : (grp & 1);                     // LDS slot of U (V at su + 2)
     const int gcol_raw = blk * TB + lcol;
-    const bool col_ok = gcol_raw < p.d;
-    const int gcol = col_ok ? gcol_raw : p.d - 1;
-    const uint32_t col_keep = col_ok ? 1u : 0u;
+    // columns past d read column d - 1: the rows and columns of the tile
+    // beyond d are never read back, so their values do not matter
+    const int gcol = gcol_raw < p.d ? gcol_raw : p.d - 1;
     const float Kc = p.K[gcol];
     const int64_t lagd = (int64_t)p.lag * p.d;
 
@@ -679,57 +680,61 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
         return (cur.base + (q - cur.p0) * p.stride) * p.d;
     };
 
-    // element offsets of the x rows of one stage's pairs (scalar registers,
-    // computed one block ahead of the loads), the raw x, y of a stage (zero
-    // for padding pairs and columns) and the validity bit of each pair
-    int64_t ra[PPT], rb[PPT];
+    // Frame loads of one stage.  Row pointers are wave-uniform (scalar
+    // registers) and the lane offset a 32-bit VGPR, so the addresses cost no
+    // vector instructions.  Common case: the stage's pairs lie in one
+    // trajectory, the rows are base + q * stride * d and the per-pair lane
+    // offsets are fixed.  Otherwise every pair gets its own row; a pair past
+    // the slice reads the shift vector K for both x and y, which makes its U
+    // and V exactly zero.
     const int64_t step_d = (int64_t)p.stride * p.d;
-    auto rows_for = [&](int64_t *r, int s) __attribute__((always_inline)) {
-        const int64_t q0 = p_lo + (int64_t)s * RBT + pfirst;
-        if (q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi) {
-            // common case: all pairs in the cursor's trajectory
-            const int64_t r0 = (cur.base + (q0 - cur.p0) * p.stride) * p.d;
+    uint32_t voff[PPT];  // byte offsets of (pair q, column gcol) from the stage's first row
 #pragma unroll
-            for (int q = 0; q < PPT; ++q) r[q] = r0 + q * step_d;
-        } else {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) r[q] = row_of(q0 + q);
-        }
+    for (int q = 0; q < PPT; ++q) voff[q] = (uint32_t)(((int64_t)q * step_d + gcol) * 4);
+    const uint32_t coff = (uint32_t)gcol * 4u;
+    const char *Xb = reinterpret_cast<const char *>(p.X);
+    const char *Kb = reinterpret_cast<const char *>(p.K);
+    // first pair of stage t, and whether all its pairs lie in the cursor's trajectory
+    auto stage_q0 = [&](int t) __attribute__((always_inline)) -> int64_t { return p_lo + (int64_t)t * RBT + pfirst; };
+    auto stage_fast = [&](int64_t q0) __attribute__((always_inline)) -> bool {
+        return q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi;
     };
     float xa[PPT], ya[PPT], xb[PPT], yb[PPT];
-    uint32_t va = 0, vb = 0;
-    // row pointers are wave-uniform (scalar), the column a 32-bit lane offset:
-    // no vector address arithmetic; values are masked when they are used
-    const uint32_t gcol_u = (uint32_t)gcol;
-    auto issue = [&](float *xr, float *yr, uint32_t &vm, const int64_t *rows) __attribute__((always_inline)) {
-        vm = 0;
+    auto issue_fast = [&](float *xr, float *yr, int64_t q0) __attribute__((always_inline)) {
+        const int64_t r0 = (cur.base + (q0 - cur.p0) * p.stride) * p.d;
+        const char *bx = Xb + r0 * 4;
+        const char *by = bx + lagd * 4;
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
-            const int64_t xo = rows[q];
-            const float *xp = p.X + (xo >= 0 ? xo : 0);
-            const float *yp = xo >= 0 ? xp + lagd : p.X;
-            xr[q] = xp[gcol_u];
-            yr[q] = yp[gcol_u];
-            vm |= (xo >= 0 ? (col_keep & 1u) : 0u) << q;
+            xr[q] = *reinterpret_cast<const float *>(bx + voff[q]);
+            yr[q] = *reinterpret_cast<const float *>(by + voff[q]);
+        }
+    };
+    auto issue_slow = [&](float *xr, float *yr, int64_t q0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int64_t r = row_of(q0 + q);
+            const char *bx = r >= 0 ? Xb + r * 4 : Kb;
+            const char *by = r >= 0 ? bx + lagd * 4 : Kb;
+            xr[q] = *reinterpret_cast<const float *>(bx + coff);
+            yr[q] = *reinterpret_cast<const float *>(by + coff);
         }
     };
     double sum_u = 0.0;
-    auto prep = [&](const float *xr, const float *yr, uint32_t vm, int buf) __attribute__((always_inline)) {
+    auto prep = [&](const float *xr, const float *yr, int buf) __attribute__((always_inline)) {
         uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
         float bu = 0.0f;
 #pragma unroll
         for (int q = 0; q < PPT; q += 2) {
-            // padding pairs and columns give U = V = 0
-            const bool ok0 = (vm >> q) & 1u, ok1 = (vm >> (q + 1)) & 1u;
-            const float u0 = ok0 ? __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc)) : 0.0f;
-            const float u1 = ok1 ? __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc)) : 0.0f;
-            const float v0 = ok0 ? __fsub_rn(xr[q], yr[q]) : 0.0f;
-            const float v1 = ok1 ? __fsub_rn(xr[q + 1], yr[q + 1]) : 0.0f;
-            bu = __fadd_rn(bu, __fadd_rn(u0, u1));
+            const float u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
+            const float u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
+            const float v0 = __fsub_rn(xr[q], yr[q]);
+            const float v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
+            if constexpr (DIAG) bu = __fadd_rn(bu, __fadd_rn(u0, u1));
             split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
             split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
         }
-        sum_u += (double)bu;
+        if constexpr (DIAG) sum_u += (double)bu;
         const int off = lcol * RPITCH + pfirst;
         if constexpr (PPT == 8) {
             *reinterpret_cast<uint4 *>(&lds.op[buf][su][0][off]) = make_uint4(uh[0], uh[1], uh[2], uh[3]);
@@ -800,16 +805,23 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                double *dst = slot + (int64_t)m * TB * TB + ((int64_t)(((wr * 2 + wc) * 4 + r * 2 + c) * 64 + lane)) * 16;
+                // element-major fragment order: one instruction covers whole cache
+                // lines; wave-uniform base + 32-bit lane offset (no 64-bit vector
+                // addresses to keep live across the loop)
+                char *base = reinterpret_cast<char *>(slot + (int64_t)m * TB * TB +
+                                                      (int64_t)((wr * 2 + wc) * 4 + r * 2 + c) * 1024);
+                uint32_t loff = (uint32_t)lane * 8u;
+                asm volatile("" : "+v"(loff));  // recomputed per flush: keeps the addresses out of the loop
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const double v = (double)acc[r][c][e];
+                    double *dst = reinterpret_cast<double *>(base + e * 512 + loff);
                     // private slot, one writer lane per element: no atomicity needed;
                     // the L2-side add keeps the old value out of registers
                     if (first_flush)
-                        dst[e] = v;
+                        *dst = v;
                     else
-                        unsafeAtomicAdd(dst + e, v);
+                        unsafeAtomicAdd(dst, v);
                 }
             }
     };
@@ -818,34 +830,41 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     // is prepared into the other buffer from register set (s + 1) & 1, which is
     // then refilled with stage s + 3 ----
     const int n_even = n_stages > 0 ? (n_stages + 1) & ~1 : 2;  // at least one (zero) span: every slot is written
-    rows_for(ra, 0);
-    issue(xa, ya, va, ra);
-    rows_for(rb, 1);
-    issue(xb, yb, vb, rb);
-    prep(xa, ya, va, 0);
-    rows_for(ra, 2);
-    issue(xa, ya, va, ra);
+    auto issue = [&](float *xr, float *yr, int t) __attribute__((always_inline)) {
+        const int64_t q0 = stage_q0(t);
+        if (stage_fast(q0)) issue_fast(xr, yr, q0); else issue_slow(xr, yr, q0);
+    };
+    issue(xa, ya, 0);
+    issue(xb, yb, 1);
+    prep(xa, ya, 0);
+    issue(xa, ya, 2);
     zero_acc();
     __syncthreads();
     bool first_flush = true;
+    // one half-iteration: MFMAs of the stage in buffer mb, preparation of the
+    // register set (xr, yr) into buffer pb, then its refill with stage t; the
+    // two load forms are separate blocks so that each is one schedulable unit
+    auto half = [&](auto act_tag, int mb, float *xr, float *yr, int pb, int t) __attribute__((always_inline)) {
+        constexpr bool ACT = decltype(act_tag)::value;
+        const int64_t q0 = stage_q0(t);
+        if (stage_fast(q0)) {
+            if constexpr (ACT) mfma(mb);
+            prep(xr, yr, pb);
+            issue_fast(xr, yr, q0);
+            if constexpr (ACT) interleave();
+        } else {
+            if constexpr (ACT) mfma(mb);
+            prep(xr, yr, pb);
+            issue_slow(xr, yr, q0);
+        }
+        __syncthreads();
+    };
     // waves without matrix work run the same loads, splits and barriers
     auto run = [&](auto act_tag) __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
         for (int s = 0; s < n_even; s += 2) {
-            // even stage s: MFMA buffer 0, prepare stage s + 1 (set b) into buffer 1
-            rows_for(rb, s + 3);
-            if constexpr (ACT) mfma(0);
-            prep(xb, yb, vb, 1);
-            issue(xb, yb, vb, rb);
-            if constexpr (ACT) interleave();
-            __syncthreads();
-            // odd stage s + 1: MFMA buffer 1, prepare stage s + 2 (set a) into buffer 0
-            rows_for(ra, s + 4);
-            if constexpr (ACT) mfma(1);
-            prep(xa, ya, va, 0);
-            issue(xa, ya, va, ra);
-            if constexpr (ACT) interleave();
-            __syncthreads();
+            half(act_tag, 0, xb, yb, 1, s + 3);  // even stage s; prepare stage s + 1
+            half(act_tag, 1, xa, ya, 0, s + 4);  // odd stage s + 1; prepare stage s + 2
             if (((s + 2) % RSPAN) == 0 || s + 2 >= n_even) {
                 if constexpr (ACT) {
                     flush(first_flush);
@@ -884,7 +903,8 @@ __device__ __forceinline__ int sym_task(const Params &p, int I, int J) {
 
 // Position of tile element (li, lj) inside a slot matrix.  Fragment layout:
 // wave (li/64, lj/64) of the 2x2 wave grid, 32x32 subtile (r, c), and inside
-// it the 32x32 MFMA output layout lane = jj + 32*((ii>>2)&1), e = 4*(ii>>3) + (ii&3).
+// it the 32x32 MFMA output layout lane = jj + 32*((ii>>2)&1), e = 4*(ii>>3) + (ii&3),
+// stored element-major (64 lanes of one element are consecutive).
 __device__ __forceinline__ int64_t slot_index(const Params &p, int li, int lj) {
     if (!p.frag) return (int64_t)li * p.tb + lj;
     const int wave = (li >> 6) * 2 + (lj >> 6);
@@ -892,7 +912,7 @@ __device__ __forceinline__ int64_t slot_index(const Params &p, int li, int lj) {
     const int ii = li & 31, jj = lj & 31;
     const int lane = jj + 32 * ((ii >> 2) & 1);
     const int e = 4 * (ii >> 3) + (ii & 3);
-    return ((int64_t)((wave * 4 + r * 2 + c) * 64 + lane)) * 16 + e;
+    return (int64_t)(wave * 4 + r * 2 + c) * 1024 + e * 64 + lane;
 }
 
 __device__ __forceinline__ double slot_sum(const Params &p, int task, int m, int li, int lj) {
