@@ -78,6 +78,8 @@ def main():
     ev = {k: torch.cuda.Event(enable_timing=True) for k in ("m0", "m1")}
     ea = {k: torch.cuda.Event(enable_timing=True) for k in ("a0", "a1")}
 
+    eig_s = []
+
     def step(prec=_native.EMMA_PREC_BF16X4, amode=_native.EMMA_ASSIGN_BF16X3_RECHECK):
         s_asg.wait_stream(s_mom)
         ev["m0"].record(s_mom)
@@ -91,7 +93,9 @@ def main():
         wh = w.item()  # waits for the moments only
         C00 = Mxx.cpu().numpy() / wh
         C0t = Mxy.cpu().numpy() / wh
+        te = time.perf_counter()
         l, R = _solvers.eig_corr(C00, C0t, 1e-6, sign_maxelement=True)
+        eig_s.append(time.perf_counter() - te)
         torch.cuda.current_stream().wait_stream(s_asg)
         torch.cuda.synchronize()
         return l, ev["m0"].elapsed_time(ev["m1"]), ea["a0"].elapsed_time(ea["a1"])
@@ -123,7 +127,10 @@ def main():
     ach = kern[dom]["tflops"]
     roofline = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_BF16_TFLOPS, "traffic": None}
-    log("step %.2f ms (moments %.2f ms, assign %.2f ms, rechecked %d)" % (step_s * 1e3, mom_ms, asg_ms, n_re))
+    eig_ms = float(np.mean(eig_s[-args.steps:])) * 1e3
+    kern["eig_host"] = {"ms": eig_ms, "threads": _solvers.blas_threads()}
+    log("step %.2f ms (moments %.2f ms, assign %.2f ms, host eig %.2f ms, rechecked %d)"
+        % (step_s * 1e3, mom_ms, asg_ms, eig_ms, n_re))
 
     # exact mode: FP64 MFMA moments (one call, same stream)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -262,11 +262,16 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     constexpr int DP = KSTEPS * 16;
     constexpr int PITCH = tile_pitch(DP);
     constexpr int IMG = tile_img_bytes(DP);
-    constexpr int CHUNKS = IMG / 16;
+    constexpr int MAT = 128 * PITCH;        // bytes of hi + lo in an image
+    constexpr int MCH = MAT / 16;           // 16-byte chunks of hi + lo
+    constexpr int CCH = 3 * 32 * 4 / 16;    // 16-byte chunks of nb, q, r
     constexpr int NT = NW * 64;
-    constexpr int ROUNDS = (CHUNKS + NT - 1) / NT;
-    // two separate objects, so that the compiler can tell the buffer being
-    // read from the one being filled
+    constexpr int ROUNDS = (MCH + NT - 1) / NT;
+    // LDS buffer b holds hi + lo of a tile t with t & 1 == b and the constants
+    // (nb, q, r) of tile t - 1: the MFMAs of tile t and the epilogue of tile
+    // t - 1 run together and read the same buffer, while the copies fill the
+    // other one.  Two separate objects, so that the compiler can tell the
+    // buffer being read from the one being filled.
     __shared__ __align__(16) uint8_t s_img0[IMG];
     __shared__ __align__(16) uint8_t s_img1[IMG];
 
@@ -278,17 +283,22 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
     const int ntiles = k_pad / 32;
 
-    // asynchronous copy of tile image t into an LDS buffer
-    auto fetch = [&](int t, uint8_t *dst) __attribute__((always_inline)) {
-        const uint8_t *src = cp.img + (int64_t)t * IMG;
+    // asynchronous copies into an LDS buffer: hi + lo of tile tm (tm < 0: none)
+    // and the constants of tile tc (tc < 0: none)
+    auto fetch = [&](int tm, int tc, uint8_t *dst) __attribute__((always_inline)) {
+        if (tm >= 0) {
+            const uint8_t *src = cp.img + (int64_t)tm * IMG;
 #pragma unroll
-        for (int i = 0; i < ROUNDS; ++i) {
-            const int c0 = i * NT + wave * 64;  // first chunk of this wave's 1 KB block
-            if (c0 + lane < CHUNKS)
-                __builtin_amdgcn_global_load_lds(src + (int64_t)(c0 + lane) * 16, (lds_ptr_t)(dst + c0 * 16), 16, 0, 0);
+            for (int i = 0; i < ROUNDS; ++i) {
+                const int c0 = i * NT + wave * 64;  // first chunk of this wave's 1 KB block
+                if (c0 + lane < MCH)
+                    __builtin_amdgcn_global_load_lds(src + (int64_t)(c0 + lane) * 16, (lds_ptr_t)(dst + c0 * 16), 16, 0, 0);
+            }
         }
+        if (tc >= 0 && wave == 0 && lane < CCH)
+            __builtin_amdgcn_global_load_lds(cp.img + (int64_t)tc * IMG + MAT + lane * 16, (lds_ptr_t)(dst + MAT), 16, 0, 0);
     };
-    fetch(0, s_img0);
+    fetch(0, -1, s_img0);
 
     // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo (branch-free) ----
     bf16x8 bhi[KSTEPS], blo[KSTEPS];
@@ -318,13 +328,15 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     const float na = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
     const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
 
-    float tmin = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
-    int imin = 0x7fffffff, ilb1 = 0x7fffffff;
+    // running best d~ (tmin at element code icode = 16 t + e), the lower bound
+    // of that candidate (lbt) and the two smallest lower bounds over all
+    // candidates (lb1 <= lb2, as a multiset)
+    float tmin = INFINITY, lbt = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
+    int icode = 0x7fffffff;
 
-    auto compute = [&](const uint8_t *img, int t) __attribute__((always_inline)) {
+    auto chain = [&](const uint8_t *img) __attribute__((always_inline)) -> floatx16 {
         const uint16_t *ah_base = reinterpret_cast<const uint16_t *>(img) + (lane & 31) * PITCH + kh;
         const uint16_t *al_base = ah_base + 32 * PITCH;
-        const float *cst = reinterpret_cast<const float *>(img + 128 * PITCH);
         floatx16 acc = {};
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
@@ -334,7 +346,10 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[s], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[s], acc, 0, 0, 0);
         }
-        // epilogue: running best d~ and the two smallest lower bounds
+        return acc;
+    };
+    auto epilogue = [&](const floatx16 &acc, const uint8_t *img, int t) __attribute__((always_inline)) {
+        const float *cst = reinterpret_cast<const float *>(img + MAT);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c0 = 8 * g + 4 * (lane >> 5);
@@ -347,55 +362,84 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const float p = acc[4 * g + h];
-                const int j = t * 32 + c0 + h;
                 const float tv = fmaf(-2.0f, p, nba[h]);
                 const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
-                const bool c1 = tv < tmin;
+                const bool c1 = tv < tmin;  // strict: the first minimum wins
                 tmin = c1 ? tv : tmin;
-                imin = c1 ? j : imin;
-                const bool c2 = lb < lb1, c3 = lb < lb2;
-                lb2 = c2 ? lb1 : (c3 ? lb : lb2);
-                ilb1 = c2 ? j : ilb1;
-                lb1 = c2 ? lb : lb1;
+                icode = c1 ? t * 16 + 4 * g + h : icode;
+                lbt = c1 ? lb : lbt;
+                lb2 = fminf(lb2, fmaxf(lb, lb1));
+                lb1 = fminf(lb1, lb);
             }
         }
     };
-
-    // tiles in pairs: buffer 0 holds even tiles, buffer 1 odd tiles.  Each
-    // barrier follows a wait for this wave's copies, so the tile is complete
-    // for every wave, and every wave has finished reading the buffer that the
-    // next copy overwrites.
-    for (int t = 0; t < ntiles; t += 2) {
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        __syncthreads();
-        if (t + 1 < ntiles) fetch(t + 1, s_img1);
-        compute(s_img0, t);
-        if (t + 1 < ntiles) {
-            __builtin_amdgcn_s_waitcnt(0x0F70);
-            __syncthreads();
-            if (t + 2 < ntiles) fetch(t + 2, s_img0);
-            compute(s_img1, t + 1);
+    // MFMAs of one tile interleaved with the epilogue of the previous one
+    auto interleave = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operand reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);  // 3 MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x002, 9, 0);  // epilogue VALU
         }
+    };
+
+    // tile 0, then tiles 1 .. ntiles - 1 with the epilogue one tile behind.
+    // Each barrier follows a wait for this wave's copies, so the buffer about
+    // to be read is complete for every wave and every wave has finished
+    // reading the buffer that the next copies overwrite.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+    fetch(ntiles > 1 ? 1 : -1, 0, s_img1);
+    floatx16 acc = chain(s_img0);
+    int t = 1;
+    while (t < ntiles) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        fetch(t + 1 < ntiles ? t + 1 : -1, t, s_img0);
+        {
+            const floatx16 nacc = chain(s_img1);
+            epilogue(acc, s_img1, t - 1);
+            interleave();
+            acc = nacc;
+        }
+        if (++t >= ntiles) break;
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        fetch(t + 1 < ntiles ? t + 1 : -1, t, s_img1);
+        {
+            const floatx16 nacc = chain(s_img0);
+            epilogue(acc, s_img0, t - 1);
+            interleave();
+            acc = nacc;
+        }
+        ++t;
     }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    epilogue(acc, (ntiles & 1) ? s_img1 : s_img0, ntiles - 1);
+
+    // center index of the element code: tile t, element e of the 32x32 output
+    // (lane l holds centers 8 (e >> 2) + 4 (l >> 5) + (e & 3))
+    auto center_of = [&](int code, int l) -> int {
+        const int tt = code >> 4, e = code & 15;
+        return tt * 32 + 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+    };
+    int imin = icode == 0x7fffffff ? 0x7fffffff : center_of(icode, lane);
 
     // ---- merge the two half-waves that hold the same frame ----
     {
         const float ot = __shfl_xor(tmin, 32, 64);
         const int oi = __shfl_xor(imin, 32, 64);
+        const float olt = __shfl_xor(lbt, 32, 64);
         const float o1 = __shfl_xor(lb1, 32, 64);
-        const int oi1 = __shfl_xor(ilb1, 32, 64);
         const float o2 = __shfl_xor(lb2, 32, 64);
         if (ot < tmin || (ot == tmin && oi < imin)) {
             tmin = ot;
             imin = oi;
+            lbt = olt;
         }
-        if (o1 < lb1) {
-            lb2 = fminf(lb1, o2);
-            lb1 = o1;
-            ilb1 = oi1;
-        } else {
-            lb2 = fminf(lb2, o1);
-        }
+        lb2 = fminf(fminf(lb2, o2), fmaxf(lb1, o1));
+        lb1 = fminf(lb1, o1);
     }
     if (lane < 32 && fvalid && imin >= k) {
         // non-finite distances: leave it to the exact loop
@@ -403,8 +447,10 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
         list[pos] = frame;
     } else if (lane < 32 && fvalid) {
-        // decision in FP64
-        const double lbo = (ilb1 == imin) ? (double)lb2 : (double)lb1;
+        // decision in FP64.  Smallest lower bound over the other candidates:
+        // if the winner's own bound is the smallest, the second smallest (a
+        // tie of two candidates at lb1 makes lb2 == lb1), otherwise lb1
+        const double lbo = (lbt == lb1) ? (double)lb2 : (double)lb1;
         const double bnb = (double)cp.bn[imin];
         const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
         const double upper = ((double)na + (double)tmin + E1) * (1.0 + (double)gref);
@@ -478,12 +524,12 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
 struct Variant {
     int nw, minb;
 };
-static const Variant kVarSmall[] = {{4, 2}, {8, 1}, {8, 2}, {16, 1}};  // KSTEPS <= 8
-static const Variant kVarMid[] = {{4, 2}, {8, 1}};                    // 12, 16, 19
-static const Variant kVarLarge[] = {{4, 1}};                          // 24, 32
+static const Variant kVarSmall[] = {{4, 2}, {8, 1}, {16, 1}};  // KSTEPS <= 4
+static const Variant kVarMid[] = {{4, 2}, {8, 1}};             // 8 .. 19
+static const Variant kVarLarge[] = {{4, 1}};                   // 24, 32
 
 static int n_variants(int ks) {
-    return ks <= 8 ? (int)(sizeof(kVarSmall) / sizeof(Variant))
+    return ks <= 4 ? (int)(sizeof(kVarSmall) / sizeof(Variant))
                    : (ks <= 19 ? (int)(sizeof(kVarMid) / sizeof(Variant)) : (int)(sizeof(kVarLarge) / sizeof(Variant)));
 }
 
@@ -522,11 +568,10 @@ template <int KS>
 static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     const int v = selected_variant(KS, k_pad);
-    if constexpr (KS <= 8) {
+    if constexpr (KS <= 4) {
         switch (v) {
             case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 2: launch_v<KS, 8, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 3: launch_v<KS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 2: launch_v<KS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
             default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
         }
     } else if constexpr (KS <= 19) {
