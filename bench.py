@@ -5,9 +5,10 @@ Workload (default): 200 trajectories x 50,000 frames x 300 float32 features
 the GPU), TICA lag 20 and assignment to k = 1000 centers (kmeans++ on a
 200k-frame subsample, outside the timed region).
 
-One step = lagged moments of all pairs (bf16 hi/lo split on MFMA, FP64 merge)
--> host FP64 eigensolve, concurrently with the assignment of every frame
-(bf16x3 MFMA + exact recheck) on a second stream.  Prints one JSON line.
+One step = lagged moments of all pairs (bf16 hi/lo split on MFMA, FP64 merge),
+then the assignment of every frame (bf16x3 MFMA + exact recheck) on a second
+stream while the host solves the FP64 eigenproblem of the moments.  Prints
+one JSON line.
 """
 import argparse
 import json
@@ -81,11 +82,14 @@ def main():
     eig_s = []
 
     def step(prec=_native.EMMA_PREC_BF16X4, amode=_native.EMMA_ASSIGN_BF16X3_RECHECK):
-        s_asg.wait_stream(s_mom)
+        # moments first (each kernel fills the GPU on its own), then the
+        # assignment on the second stream while the host solves the d x d
+        # eigenproblem of the moments
         ev["m0"].record(s_mom)
         w, sx, sy, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True,
                                                         c0t=True, precision=prec)
         ev["m1"].record(s_mom)
+        s_asg.wait_stream(s_mom)
         with torch.cuda.stream(s_asg):
             ea["a0"].record(s_asg)
             _native.assign(X, C, mode=amode, labels=labels, n_rechecked=nre)

@@ -47,6 +47,7 @@ extern "C" {
 #define EMMA_MOM_REMOVE_MEAN 2u  /* mean-free (co)moments */
 #define EMMA_MOM_C0T 4u          /* compute Mxy */
 #define EMMA_MOM_CTT 8u          /* compute Myy = Y'Y (not with SYMMETRIZE) */
+#define EMMA_MOM_WEIGHTED 16u    /* frame weights given (emma_moments_lagged_weighted) */
 
 /* arithmetic of the covariance tiles */
 #define EMMA_PREC_FP32 0    /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 tile accumulation */
@@ -99,6 +100,20 @@ int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, const int64_t
                         int32_t precision, double *w, double *sx, double *sy, double *Mxx,
                         double *Mxy, double *Myy, void *workspace, size_t workspace_bytes,
                         void *stream);
+
+/*
+ * The same moments with frame weights: row_weights (device FP64, one per row
+ * of X) weigh every term of a pair by the weight of its x frame, and w becomes
+ * the weight sum (times 2 with SYMMETRIZE), as the reference's weighted
+ * estimator does (moments.py:267-303, 540-553; weights iterated with the data,
+ * covariance.py:198-240).  flags must include EMMA_MOM_WEIGHTED; the workspace
+ * size is queried with the same flags.
+ */
+int emma_moments_lagged_weighted(const float *X, int64_t n_rows, int32_t d, const int64_t *traj_offsets,
+                                 int32_t ntraj, int32_t lag, int32_t stride, int32_t skip, uint32_t flags,
+                                 int32_t precision, const double *row_weights, double *w, double *sx,
+                                 double *sy, double *Mxx, double *Mxy, double *Myy, void *workspace,
+                                 size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Assignment                                                          */
@@ -192,6 +207,14 @@ int emma_lloyd(const float *X, int64_t n, int32_t d, float *C, int32_t k, int32_
  */
 int emma_project(const float *X, int64_t n, int32_t d, const double *mean, const double *V,
                  int32_t ldv, int32_t m, float *Y, void *stream);
+
+/*
+ * out[i] = sum_c X[i,c] u[c] + u0 in FP64 (out device FP64, n entries): the
+ * frame weights of a linear weight model, _KoopmanWeights.weights
+ * (coordinates/estimation/koopman.py:54-60).
+ */
+int emma_row_affine(const float *X, int64_t n, int32_t d, const double *u, double u0, double *out,
+                    void *stream);
 
 /*
  * count[0] = number of non-finite entries of X (n elements, float32).

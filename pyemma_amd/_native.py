@@ -18,6 +18,7 @@ EMMA_MOM_SYMMETRIZE = 1
 EMMA_MOM_REMOVE_MEAN = 2
 EMMA_MOM_C0T = 4
 EMMA_MOM_CTT = 8
+EMMA_MOM_WEIGHTED = 16
 EMMA_PREC_FP32 = 0
 EMMA_PREC_BF16X4 = 1
 EMMA_PREC_FP64 = 2
@@ -50,6 +51,11 @@ def lib():
     L.emma_moments_lagged.restype = ctypes.c_int
     L.emma_moments_lagged.argtypes = [vp, i64, i32, vp, i32, i32, i32, i32, u32, i32, vp, vp, vp, vp, vp, vp,
                                       vp, sz, vp]
+    L.emma_moments_lagged_weighted.restype = ctypes.c_int
+    L.emma_moments_lagged_weighted.argtypes = [vp, i64, i32, vp, i32, i32, i32, i32, u32, i32, vp, vp, vp, vp, vp,
+                                               vp, vp, vp, sz, vp]
+    L.emma_row_affine.restype = ctypes.c_int
+    L.emma_row_affine.argtypes = [vp, i64, i32, vp, dbl, vp, vp]
     L.emma_assign_workspace_size.restype = sz
     L.emma_assign_workspace_size.argtypes = [i64, i32, i32, i32]
     L.emma_assign.restype = ctypes.c_int
@@ -147,8 +153,11 @@ def count_nonfinite(Xt):
 
 
 def moments_lagged(Xt, traj_offsets, lag, stride=1, skip=0, reversible=True, remove_mean=True, c0t=True,
-                   ctt=False, precision=EMMA_PREC_BF16X4):
-    """Unnormalized FP64 moments (w, sx, sy, Mxx, Mxy, Myy) as device tensors."""
+                   ctt=False, precision=EMMA_PREC_BF16X4, weights=None):
+    """Unnormalized FP64 moments (w, sx, sy, Mxx, Mxy, Myy) as device tensors.
+
+    weights: optional FP64 device tensor with one frame weight per row of Xt.
+    """
     t = torch()
     n_rows, d = Xt.shape
     off = traj_offsets
@@ -164,6 +173,11 @@ def moments_lagged(Xt, traj_offsets, lag, stride=1, skip=0, reversible=True, rem
         flags |= EMMA_MOM_C0T
     if ctt and lag > 0:
         flags |= EMMA_MOM_CTT
+    if weights is not None:
+        flags |= EMMA_MOM_WEIGHTED
+        if weights.dtype != t.float64 or weights.numel() != n_rows or weights.device != Xt.device:
+            raise ValueError("weights must be a float64 device tensor with one entry per frame")
+        weights = weights.contiguous()
     L = lib()
     ws_bytes = L.emma_moments_workspace_size(n_rows, d, ntraj, flags, precision)
     ws = workspace(ws_bytes, "moments")
@@ -174,9 +188,10 @@ def moments_lagged(Xt, traj_offsets, lag, stride=1, skip=0, reversible=True, rem
     Mxx = t.empty((d, d), **f64)
     Mxy = t.empty((d, d), **f64) if flags & EMMA_MOM_C0T else None
     Myy = t.empty((d, d), **f64) if flags & EMMA_MOM_CTT else None
-    check(L.emma_moments_lagged(ptr(Xt), n_rows, d, ptr(off), ntraj, int(lag), int(stride), int(skip), flags,
-                                int(precision), ptr(w), ptr(sx), ptr(sy), ptr(Mxx), ptr(Mxy), ptr(Myy),
-                                ptr(ws), ws.numel(), stream_ptr()), "emma_moments_lagged")
+    check(L.emma_moments_lagged_weighted(ptr(Xt), n_rows, d, ptr(off), ntraj, int(lag), int(stride), int(skip),
+                                         flags, int(precision), ptr(weights), ptr(w), ptr(sx), ptr(sy), ptr(Mxx),
+                                         ptr(Mxy), ptr(Myy), ptr(ws), ws.numel(), stream_ptr()),
+          "emma_moments_lagged")
     return w, sx, sy, Mxx, Mxy, Myy
 
 
@@ -232,3 +247,15 @@ def project(Xt, mean_dev, V_dev, m):
     check(lib().emma_project(ptr(Xt), n, d, ptr(mean_dev), ptr(V_dev), ldv, int(m), ptr(Y), stream_ptr()),
           "emma_project")
     return Y
+
+
+def row_affine(Xt, u, u0):
+    """out[i] = X[i] . u + u0 in FP64 (device tensor)."""
+    t = torch()
+    n, d = Xt.shape
+    u_dev = t.as_tensor(np.asarray(u, dtype=np.float64).ravel(), device=Xt.device)
+    if u_dev.numel() != d:
+        raise ValueError("weight vector has %d entries, data has %d columns" % (u_dev.numel(), d))
+    out = t.empty(n, dtype=t.float64, device=Xt.device)
+    check(lib().emma_row_affine(ptr(Xt), n, d, ptr(u_dev), float(u0), ptr(out), stream_ptr()), "emma_row_affine")
+    return out

@@ -17,36 +17,47 @@ from .tica import TICA
 __all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers"]
 
 
-def _weights_arg(weights, data, reversible=True, for_tica=True):
-    """api.py:1252-1272 / 1634-1652: only the empirical (unweighted) estimator runs on the GPU path."""
+def _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=True, for_tica=True, precision="fp64"):
+    """api.py:1252-1272 (tica) / 1633-1650 (covariance_lagged): resolve the weights argument.
+
+    'koopman' runs the Koopman estimator (a non-reversible moments pass on the
+    GPU and a small host eigenproblem) and returns its weight model.
+    """
+    from .koopman import KoopmanEstimator
+
     if isinstance(weights, str):
         if weights == "koopman":
             if data is None:
                 raise ValueError("Data must be supplied for reweighting='koopman'")
             if for_tica and not reversible:
                 raise ValueError("Koopman re-weighting is designed for reversible processes, set reversible=True")
-            raise NotImplementedError("weights='koopman' is not implemented on the GPU path yet")
+            koop = KoopmanEstimator(lag=lag, stride=stride, skip=skip, ncov_max=ncov_max, precision=precision)
+            koop.estimate(data)
+            return koop.weights
         if weights == "empirical":
             return None
         raise ValueError("reweighting must be either 'empirical', 'koopman' or an object with a weights(data) "
                          "method.")
     if hasattr(weights, "weights") and isinstance(getattr(weights, "weights"), types.MethodType):
-        raise NotImplementedError("weight objects are not implemented on the GPU path yet")
-    if isinstance(weights, (list, tuple, np.ndarray)):
+        return weights
+    if isinstance(weights, (list, tuple)) and (not for_tica or all(isinstance(w, np.ndarray) for w in weights)):
         if for_tica and data is not None and len(data) != len(weights):
             raise ValueError("len of weights({}) must match len of data({}).".format(len(weights), len(data)))
-        raise NotImplementedError("per-frame weights are not implemented on the GPU path yet")
+        return weights
+    if isinstance(weights, np.ndarray) and not for_tica:
+        return weights
     raise ValueError("reweighting must be either 'empirical', 'koopman' or an object with a weights(data) method.")
 
 
 def tica(data=None, lag=10, dim=-1, var_cutoff=0.95, kinetic_map=True, commute_map=False, weights="empirical",
          stride=1, remove_mean=True, skip=0, reversible=True, ncov_max=float("inf"), chunksize=None, **kwargs):
     r"""Time-lagged independent component analysis (api.py:1046-1290)."""
-    weights = _weights_arg(weights, data, reversible=reversible, for_tica=True)
+    precision = kwargs.pop("precision", "bf16x4")
+    weights = _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=reversible, for_tica=True,
+                           precision=precision)
     if not remove_mean:
         warnings.warn("remove_mean option is deprecated. The mean is removed from the data by default, otherwise "
                       "it cannot be guaranteed that all eigenvalues will be smaller than one.", DeprecationWarning)
-    precision = kwargs.pop("precision", "bf16x4")
     res = TICA(lag, dim=dim, var_cutoff=var_cutoff, kinetic_map=kinetic_map, commute_map=commute_map, skip=skip,
                stride=stride, weights=weights, reversible=reversible, ncov_max=ncov_max, precision=precision)
     if data is not None:
@@ -60,12 +71,12 @@ def covariance_lagged(data=None, c00=True, c0t=True, ctt=False, remove_constant_
                       reversible=False, bessel=True, lag=0, weights="empirical", stride=1, skip=0, chunksize=None,
                       ncov_max=float("inf"), column_selection=None, diag_only=False, **kwargs):
     r"""Lagged covariances between time series (api.py:1554-1661)."""
-    weights = _weights_arg(weights, data, for_tica=False)
+    precision = kwargs.pop("precision", "bf16x4")
+    weights = _weights_arg(weights, data, lag, stride, skip, ncov_max, for_tica=False, precision=precision)
     lc = LaggedCovariance(c00=c00, c0t=c0t, ctt=ctt, remove_constant_mean=remove_constant_mean,
                           remove_data_mean=remove_data_mean, reversible=reversible, bessel=bessel, lag=lag,
                           weights=weights, stride=stride, skip=skip, ncov_max=ncov_max,
-                          column_selection=column_selection, diag_only=diag_only,
-                          precision=kwargs.pop("precision", "bf16x4"))
+                          column_selection=column_selection, diag_only=diag_only, precision=precision)
     if data is not None:
         lc.estimate(data, chunksize=chunksize)
     else:

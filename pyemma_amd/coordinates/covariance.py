@@ -131,9 +131,6 @@ class LaggedCovariance(object):
         self.chunksize = None
         self._rc = None
         self._estimated = False
-        if weights is not None:
-            raise NotImplementedError("frame weights (weights='koopman', arrays or weight objects) are not "
-                                      "supported by the GPU moments path yet")
 
     # ------------------------------------------------------------------
     def _init_storage(self, n_chunks):
@@ -146,11 +143,16 @@ class LaggedCovariance(object):
         }
 
     def _device_moments(self, dt):
+        from .koopman import frame_weights
+
         lag = self.lag
+        wts = frame_weights(self.weights, dt)
+        if wts is not None and self.ctt:
+            raise NotImplementedError("weighted Ctt is not supported")
         w, sx, sy, Mxx, Mxy, Myy = _native.moments_lagged(
             dt.X, dt.offsets_dev, lag, stride=self.stride, skip=self.skip, reversible=self.reversible,
             remove_mean=self.remove_data_mean, c0t=self.c0t, ctt=self.ctt,
-            precision=PRECISIONS[self.precision])
+            precision=PRECISIONS[self.precision], weights=wts)
         h = lambda t: None if t is None else t.cpu().numpy()
         w = float(w.item())
         sx, sy, Mxx, Mxy, Myy = h(sx), h(sy), h(Mxx), h(Mxy), h(Myy)

@@ -107,3 +107,28 @@ extern "C" int emma_project(const float *X, int64_t n, int32_t d, const double *
     k_project<<<(unsigned)cdiv(n, PROJ_ROWS), 256, lds, s>>>(X, n, d, mean, V, ldv, m, Y);
     return check_launch("emma_project");
 }
+
+// ------------------------------------------------------------------
+// row affine map: out[i] = X[i,:] . u + u0 in FP64, one wavefront per row
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_row_affine(const float *__restrict__ X, int64_t n, int d,
+                                                    const double *__restrict__ u, double u0,
+                                                    double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const float *x = X + row * d;
+    double a = 0.0;
+    for (int c = lane; c < d; c += 64) a = fma((double)x[c], u[c], a);
+    a = emma::wave_butterfly_sum(a);
+    if (lane == 0) out[row] = a + u0;
+}
+
+extern "C" int emma_row_affine(const float *X, int64_t n, int32_t d, const double *u, double u0, double *out,
+                               void *stream) {
+    EMMA_REQUIRE(n >= 0 && d > 0, "emma_row_affine: bad sizes n=%lld d=%d", (long long)n, d);
+    if (n == 0) return EMMA_OK;
+    EMMA_REQUIRE(X && u && out, "emma_row_affine: NULL pointer");
+    k_row_affine<<<(unsigned)cdiv(n, 4), 256, 0, (hipStream_t)stream>>>(X, n, d, u, u0, out);
+    return check_launch("row affine");
+}

@@ -64,6 +64,9 @@ struct Params {
     int64_t *P_total;     // [1]
     int64_t *pair_off;    // [ntraj + 1]
     float *K;             // [d]
+    const double *wts;    // per-row frame weights (nullptr: unweighted)
+    double *wpart;        // [ntraj] weight sum of each trajectory's pairs
+    double *W;            // [1] total pair weight
     double *slots;        // n_tasks * n_slices * slot_doubles
 };
 
@@ -162,7 +165,7 @@ struct Cfg<EMMA_PREC_FP64> {
     static constexpr int PARTS = 1;
 };
 
-template <int PREC, int KIND, int NMAT>
+template <int PREC, int KIND, int NMAT, bool WGT>
 __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_launch_tasks) {
     using C = Cfg<PREC>;
     constexpr int TB = C::TB;
@@ -176,6 +179,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     __shared__ __align__(16) elem s_op[4][C::PARTS][TB * PITCH];  // slots A1, B1, A2, B2
     __shared__ int64_t s_xo[2][BT];                               // element offset of the x row (-1: none)
     __shared__ int s_tr[2][BT];                                   // trajectory of each pair
+    __shared__ double s_w[2][BT];                                 // weight of each pair (WGT)
     __shared__ double s_sum[2][TPC][TB];                          // column-sum partials (diagonal tasks)
 
     // XCD-aware numbering: workgroups are dealt round-robin to the 8 XCDs, so
@@ -217,8 +221,8 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
     // must intersect the valid d x d region and, on a diagonal tile, lie on or
     // above the diagonal.  (Per-subtile predicates inside the stage loop make
     // the compiler shuttle the accumulators between register files.)
-    constexpr int WT = TB / 2;  // wave tile edge
-    const bool wave_act = (I * TB + wr * WT < p.d) && (J * TB + wc * WT < p.d) && !(diag && wr > wc);
+    constexpr int WTE = TB / 2;  // wave tile edge
+    const bool wave_act = (I * TB + wr * WTE < p.d) && (J * TB + wc * WTE < p.d) && !(diag && wr > wc);
 
     // ---- pair table: offsets of the x rows of one stage ----
     auto build_table = [&](int b, int64_t s0, int t_start) __attribute__((always_inline)) {
@@ -226,12 +230,16 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
             const int64_t q = s0 + tid;
             int t = t_start;
             int64_t xo = -1;
+            double wq = 0.0;
             if (q < p_hi) {
                 while (p.pair_off[t + 1] <= q) ++t;
-                xo = (p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride) * p.d;
+                const int64_t row = p.traj_off[t] + p.skip + (q - p.pair_off[t]) * p.stride;
+                xo = row * p.d;
+                if constexpr (WGT) wq = p.wts[row];
             }
             s_xo[b][tid] = xo;
             s_tr[b][tid] = t;
+            if constexpr (WGT) s_w[b][tid] = wq;
         }
     };
     if (tid == 0) {
@@ -248,11 +256,13 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 
     // ---- prefetch registers ----
     float xr[PPT], yr[PPT];
+    double wgt[WGT ? PPT : 1];  // pair weights of the prefetched stage
     uint32_t vmask = 0;
     auto issue_loads = [&](int b) __attribute__((always_inline)) {
         vmask = 0;
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
+            if constexpr (WGT) wgt[q] = s_w[b][pq * PPT + q];
             const int64_t xo = s_xo[b][pq * PPT + q];
             const bool v = xo >= 0;
             vmask |= (v ? 1u : 0u) << q;
@@ -278,35 +288,39 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
             const bool ok = ((vmask >> q) & 1u) && col_ok;
             const float x = ok ? xr[q] : 0.0f;
             const float y = ok ? yr[q] : 0.0f;
+            // frame weights multiply the A operand (block I loaders) and the sums
             if constexpr (F64) {
                 const double xs = ok ? (double)x - (double)Kc : 0.0;
                 const double ys = ok ? (double)y - (double)Kc : 0.0;
-                sum_x += xs;
-                sum_y += ys;
+                const double wa = (WGT && half == 0) ? wgt[WGT ? q : 0] : 1.0;
+                sum_x += WGT ? wgt[WGT ? q : 0] * xs : xs;
+                sum_y += WGT ? wgt[WGT ? q : 0] * ys : ys;
                 if constexpr (KIND == REV) {
-                    d1[q] = xs + ys;
-                    d2[q] = (double)x - (double)y;
+                    d1[q] = wa * (xs + ys);
+                    d2[q] = wa * ((double)x - (double)y);
                 } else if constexpr (KIND == NR_XY) {
-                    d1[q] = half == 0 ? xs : ys;
+                    d1[q] = half == 0 ? wa * xs : ys;
                     d2[q] = 0.0;
                 } else {
-                    d1[q] = xs;
-                    d2[q] = ys;
+                    d1[q] = wa * xs;
+                    d2[q] = wa * ys;
                 }
             } else {
                 const float xs = ok ? x - Kc : 0.0f;
                 const float ys = ok ? y - Kc : 0.0f;
-                bx += xs;
-                by += ys;
+                const float wq = WGT ? (float)wgt[WGT ? q : 0] : 1.0f;
+                const float wa = (WGT && half == 0) ? wq : 1.0f;
+                bx += WGT ? wq * xs : xs;
+                by += WGT ? wq * ys : ys;
                 if constexpr (KIND == REV) {
-                    o1[q] = xs + ys;
-                    o2[q] = x - y;
+                    o1[q] = wa * (xs + ys);
+                    o2[q] = wa * (x - y);
                 } else if constexpr (KIND == NR_XY) {
-                    o1[q] = half == 0 ? xs : ys;
+                    o1[q] = half == 0 ? wa * xs : ys;
                     o2[q] = 0.0f;
                 } else {
-                    o1[q] = xs;
-                    o2[q] = ys;
+                    o1[q] = wa * xs;
+                    o2[q] = wa * ys;
                 }
             }
         }
@@ -893,6 +907,34 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
 }
 
 // ------------------------------------------------------------------
+// frame weights: sum of the weights of every trajectory's pairs, then the
+// total (both in a fixed order)
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wsum(Params p) {
+    __shared__ double s_part[256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int64_t np = p.pair_off[t + 1] - p.pair_off[t];
+    const int64_t base = p.traj_off[t] + p.skip;
+    double a = 0.0;
+    for (int64_t i = tid; i < np; i += 256) a += p.wts[base + i * p.stride];
+    s_part[tid] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) s_part[tid] += s_part[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) p.wpart[t] = s_part[0];
+}
+
+__global__ void k_wtotal(Params p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double a = 0.0;
+        for (int t = 0; t < p.ntraj; ++t) a += p.wpart[t];
+        *p.W = a;
+    }
+}
+
+// ------------------------------------------------------------------
 // reduction + FP64 corrections
 // ------------------------------------------------------------------
 __device__ __forceinline__ int sym_task(const Params &p, int I, int J) {
@@ -927,7 +969,8 @@ __global__ __launch_bounds__(256) void k_reduce_sums(Params p, double *Sx, doubl
                                                      double *sx_out, double *sy_out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t P = *p.P_total;
-    const double w = p.rev ? 2.0 * (double)P : (double)P;
+    const double Wp = p.wts ? *p.W : (double)P;  // total pair weight
+    const double w = p.rev ? 2.0 * Wp : Wp;
     if (c == 0) *w_out = w;
     if (c >= p.d) return;
     const int I = c / p.tb, lc = c - I * p.tb;
@@ -949,8 +992,8 @@ __global__ __launch_bounds__(256) void k_reduce_sums(Params p, double *Sx, doubl
         sx_out[c] = s;
         if (sy_out) sy_out[c] = s;
     } else {
-        sx_out[c] = ax + (double)P * K;
-        if (sy_out) sy_out[c] = ay + (double)P * K;
+        sx_out[c] = ax + Wp * K;
+        if (sy_out) sy_out[c] = ay + Wp * K;
     }
 }
 
@@ -970,6 +1013,7 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
     const int ts = sym_task(p, Is, Js);
     const int lis = is - Is * tb, ljs = js - Js * tb;
     const double K_i = (double)p.K[i], K_j = (double)p.K[j];
+    const double Wp = p.wts ? *p.W : (double)P;  // total pair weight
     if (P == 0) {
         if (Mxx) Mxx[e] = 0.0;
         if (Mxy) Mxy[e] = 0.0;
@@ -978,14 +1022,14 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
     }
     if (p.lag == 0) {
         const double raw = slot_sum(p, ts, 0, lis, ljs);
-        const double w = (double)P;
+        const double w = Wp;
         Mxx[e] = remove_mean ? raw - Sx[i] * Sx[j] / w : raw + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
         return;
     }
     if (p.rev) {
         const double uu = slot_sum(p, ts, 0, lis, ljs);
         const double vv = slot_sum(p, ts, 1, lis, ljs);
-        const double w = 2.0 * (double)P;
+        const double w = 2.0 * Wp;
         const double Si = Sx[i] + Sy[i], Sj = Sx[j] + Sy[j];
         const double rxx = 0.5 * (uu + vv), rxy = 0.5 * (uu - vv);
         const double corr = remove_mean ? -Si * Sj / w : Si * K_j + K_i * Sj + w * K_i * K_j;
@@ -993,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
         if (Mxy) Mxy[e] = rxy + corr;
         return;
     }
-    const double w = (double)P;
+    const double w = Wp;
     const double rxx = slot_sum(p, ts, 0, lis, ljs);
     Mxx[e] = remove_mean ? rxx - Sx[i] * Sx[j] / w : rxx + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
     if (Myy) {
@@ -1007,11 +1051,11 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
     }
 }
 
-template <int PREC>
-static void launch_kind(const Params &p, hipStream_t s) {
+template <int PREC, bool WT>
+static void launch_kind_w(const Params &p, hipStream_t s) {
     const int n_sym = p.n_sym;
     const unsigned g_sym = (unsigned)(cdiv((int64_t)n_sym * p.n_slices, 8) * 8);
-    if constexpr (PREC == EMMA_PREC_BF16X4) {
+    if constexpr (PREC == EMMA_PREC_BF16X4 && !WT) {
         if (p.rev) {
             const unsigned g_diag = (unsigned)(cdiv((int64_t)p.nb * p.n_slices, 8) * 8);
             k_rev_bf16<true><<<g_diag, RNT, 0, s>>>(p);
@@ -1023,18 +1067,26 @@ static void launch_kind(const Params &p, hipStream_t s) {
         }
     }
     if (p.lag == 0)
-        k_main<PREC, LAG0, 1><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        k_main<PREC, LAG0, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
     else if (p.rev)
-        k_main<PREC, REV, 2><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        k_main<PREC, REV, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
     else if (p.ctt)
-        k_main<PREC, NR_SYM, 2><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        k_main<PREC, NR_SYM, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
     else
-        k_main<PREC, NR_SYM, 1><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        k_main<PREC, NR_SYM, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
     if (p.has_xy) {
         const int n_xy = p.nb * p.nb;
         const unsigned g_xy = (unsigned)(cdiv((int64_t)n_xy * p.n_slices, 8) * 8);
-        k_main<PREC, NR_XY, 1><<<g_xy, NTHR, 0, s>>>(p, n_sym, n_xy);
+        k_main<PREC, NR_XY, 1, WT><<<g_xy, NTHR, 0, s>>>(p, n_sym, n_xy);
     }
+}
+
+template <int PREC>
+static void launch_kind(const Params &p, hipStream_t s) {
+    if (p.wts)
+        launch_kind_w<PREC, true>(p, s);
+    else
+        launch_kind_w<PREC, false>(p, s);
 }
 
 static int launch_main(const Params &p, int precision, hipStream_t s) {
@@ -1073,7 +1125,8 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.ctt = (flags & EMMA_MOM_CTT) && lag > 0 && !p.rev ? 1 : 0;
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
-    if (precision == EMMA_PREC_BF16X4 && p.rev) {
+    const bool weighted = (flags & EMMA_MOM_WEIGHTED) != 0;
+    if (precision == EMMA_PREC_BF16X4 && p.rev && !weighted) {
         // one 8-wave workgroup per CU: 256 slices make every launch of
         // k_rev_bf16 a whole number of rounds over the 256 CUs
         const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
@@ -1092,6 +1145,9 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     pl.Sx = cv.take<double>(d);
     pl.Sy = cv.take<double>(d);
     p.slots = cv.take<double>((size_t)p.n_tasks * p.n_slices * p.slot_doubles);
+    p.wts = nullptr;
+    p.wpart = weighted ? cv.take<double>((size_t)(ntraj > 0 ? ntraj : 1)) : nullptr;
+    p.W = weighted ? cv.take<double>(1) : nullptr;
     pl.ws_bytes = cv.bytes();
 }
 
@@ -1110,16 +1166,18 @@ extern "C" size_t emma_moments_workspace_size(int64_t n_rows, int32_t d, int32_t
     return pl.ws_bytes > pl0.ws_bytes ? pl.ws_bytes : pl0.ws_bytes;
 }
 
-extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, const int64_t *traj_offsets,
-                                   int32_t ntraj, int32_t lag, int32_t stride, int32_t skip, uint32_t flags,
-                                   int32_t precision, double *w, double *sx, double *sy, double *Mxx,
-                                   double *Mxy, double *Myy, void *workspace, size_t workspace_bytes,
-                                   void *stream) {
+extern "C" int emma_moments_lagged_weighted(const float *X, int64_t n_rows, int32_t d, const int64_t *traj_offsets,
+                                            int32_t ntraj, int32_t lag, int32_t stride, int32_t skip, uint32_t flags,
+                                            int32_t precision, const double *row_weights, double *w, double *sx,
+                                            double *sy, double *Mxx, double *Mxy, double *Myy, void *workspace,
+                                            size_t workspace_bytes, void *stream) {
     EMMA_REQUIRE(d > 0 && ntraj > 0 && n_rows >= 0, "emma_moments_lagged: bad sizes d=%d ntraj=%d", d, ntraj);
     EMMA_REQUIRE(lag >= 0 && stride >= 1 && skip >= 0, "emma_moments_lagged: bad lag/stride/skip");
     EMMA_REQUIRE(precision == EMMA_PREC_FP32 || precision == EMMA_PREC_BF16X4 || precision == EMMA_PREC_FP64,
                  "emma_moments_lagged: unknown precision %d", precision);
     EMMA_REQUIRE(X && traj_offsets && w && sx && Mxx, "emma_moments_lagged: NULL pointer");
+    EMMA_REQUIRE(((flags & EMMA_MOM_WEIGHTED) != 0) == (row_weights != nullptr),
+                 "emma_moments_lagged: EMMA_MOM_WEIGHTED and row_weights must come together");
     const bool rev = (flags & EMMA_MOM_SYMMETRIZE) && lag > 0;
     EMMA_REQUIRE(!((flags & EMMA_MOM_CTT) && rev), "emma_moments_lagged: CTT with SYMMETRIZE is meaningless");
     EMMA_REQUIRE(lag > 0 || !(flags & (EMMA_MOM_C0T | EMMA_MOM_CTT)),
@@ -1131,10 +1189,16 @@ extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, co
     EMMA_REQUIRE(workspace && workspace_bytes >= pl.ws_bytes,
                  "emma_moments_lagged: workspace too small (%zu < %zu)", workspace_bytes, pl.ws_bytes);
     mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, workspace);
+    pl.p.wts = row_weights;
     hipStream_t s = (hipStream_t)stream;
     const mom::Params &p = pl.p;
     mom::k_prep<<<1, 1024, 0, s>>>(p);
     EMMA_TRY(check_launch("moments prep"));
+    if (p.wts) {
+        mom::k_wsum<<<(unsigned)ntraj, 256, 0, s>>>(p);
+        mom::k_wtotal<<<1, 64, 0, s>>>(p);
+        EMMA_TRY(check_launch("moments weight sums"));
+    }
     EMMA_TRY(mom::launch_main(p, precision, s));
     mom::k_reduce_sums<<<(unsigned)cdiv(d, 256), 256, 0, s>>>(p, pl.Sx, pl.Sy, w, sx, sy);
     EMMA_TRY(check_launch("moments reduce sums"));
@@ -1142,4 +1206,14 @@ extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, co
     mom::k_reduce_mats<<<(unsigned)cdiv((int64_t)d * d, 256), 256, 0, s>>>(
         p, pl.Sx, pl.Sy, rm, Mxx, p.lag > 0 && (flags & EMMA_MOM_C0T) ? Mxy : nullptr, p.ctt ? Myy : nullptr);
     return check_launch("moments reduce");
+}
+
+extern "C" int emma_moments_lagged(const float *X, int64_t n_rows, int32_t d, const int64_t *traj_offsets,
+                                   int32_t ntraj, int32_t lag, int32_t stride, int32_t skip, uint32_t flags,
+                                   int32_t precision, double *w, double *sx, double *sy, double *Mxx,
+                                   double *Mxy, double *Myy, void *workspace, size_t workspace_bytes,
+                                   void *stream) {
+    EMMA_REQUIRE(!(flags & EMMA_MOM_WEIGHTED), "emma_moments_lagged: use emma_moments_lagged_weighted for weights");
+    return emma_moments_lagged_weighted(X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision,
+                                        nullptr, w, sx, sy, Mxx, Mxy, Myy, workspace, workspace_bytes, stream);
 }

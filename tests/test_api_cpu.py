@@ -35,8 +35,10 @@ def test_covariance_validation():
         LaggedCovariance(column_selection=np.array([0]), diag_only=True)
     with pytest.raises(ValueError):
         LaggedCovariance(c0t=True, ctt=True, lag=1, reversible=True)
-    with pytest.raises(NotImplementedError):
-        api.covariance_lagged(np.zeros((10, 2)), lag=1, weights=np.ones(10))
+    with pytest.raises(ValueError):
+        api.covariance_lagged(np.zeros((10, 2)), lag=1, weights="nonsense")
+    with pytest.raises(ValueError):
+        api.covariance_lagged(np.zeros((10, 2)), lag=1, weights=3.5j)
 
 
 def test_weights_arguments():
@@ -46,6 +48,17 @@ def test_weights_arguments():
         api.tica(lag=1, weights="koopman")
     with pytest.raises(ValueError):
         api.tica(np.zeros((10, 2)), lag=1, weights="koopman", reversible=False)
+    with pytest.raises(ValueError):
+        api.tica([np.zeros((10, 2))] * 2, lag=1, weights=[np.ones(10)])
+
+
+def test_koopman_u_closed_form():
+    """koopman.py:28-45 on a 2-state chain: the stationary weights of the PC|1 basis."""
+    from pyemma_amd.coordinates.koopman import _compute_u
+    K = np.array([[0.5, 0.0], [0.1, 1.0]])
+    u = _compute_u(K)
+    assert u[-1] == 1.0
+    np.testing.assert_allclose(K.T.dot(u), u, atol=1e-12)
 
 
 def test_assign_and_kmeans_validation():
