@@ -72,6 +72,10 @@ def main():
     torch.cuda.synchronize()
     log("centers: kmeans++ k=%d on %d frames" % (args.k, sub.shape[0]))
 
+    from pyemma_amd import autotune
+    best = autotune.tune_assign(X, C)
+    log("assign tile variant %s (ms per variant on a sample: %s)" % (best, autotune.tuned()))
+
     s_mom = torch.cuda.current_stream()
     s_asg = torch.cuda.Stream()
     labels = torch.empty(N, dtype=torch.int32, device=dev)
@@ -125,7 +129,7 @@ def main():
     kern = {
         "moments": {"ms": mom_ms, "algorithmic_tflop": f_mom / 1e12, "tflops": f_mom / (mom_ms * 1e-3) / 1e12},
         "assign": {"ms": asg_ms, "algorithmic_tflop": f_asg / 1e12, "tflops": f_asg / (asg_ms * 1e-3) / 1e12,
-                   "rechecked_frames": n_re},
+                   "rechecked_frames": n_re, "tile_variant": best},
     }
     dom = "assign" if asg_ms >= mom_ms else "moments"
     ach = kern[dom]["tflops"]
