@@ -24,6 +24,26 @@ sys.path.insert(0, ROOT)
 METRIC = "frames/s TICA+assign (d=300,k=1000) vs plain GPU & CPU; serving p99 ms"
 PEAK_BF16_TFLOPS = 2516.6
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_d300_k1000.md")
+
+
+def measured_traffic(kernel, default_workload):
+    """HBM bytes per call of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a plain copy), or None when the summary does not cover it or
+    the run is not the profiled workload."""
+    if not default_workload:
+        return None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes:"):
+                    for item in line.split(":", 1)[1].split(";"):
+                        name, _, val = item.strip().partition("=")
+                        if name == kernel:
+                            return int(val)
+    except OSError:
+        return None
+    return None
 
 
 def parse():
@@ -61,6 +81,7 @@ def main():
     from pyemma_amd.synthetic import brownian_distances
 
     dev = _native.device()
+    default_workload = (args.ntraj, args.nframes, args.k, args.lag) == (200, 50000, 1000, 20)
     t0 = time.time()
     X, offsets = brownian_distances(args.ntraj, args.nframes, seed=2, device=dev)
     torch.cuda.synchronize()
@@ -134,7 +155,7 @@ def main():
     dom = "assign" if asg_ms >= mom_ms else "moments"
     ach = kern[dom]["tflops"]
     roofline = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_BF16_TFLOPS, "traffic": None}
+                "frac": ach / PEAK_BF16_TFLOPS, "traffic": measured_traffic(dom, default_workload)}
     eig_ms = float(np.mean(eig_s[-args.steps:])) * 1e3
     kern["eig_host"] = {"ms": eig_ms, "threads": _solvers.blas_threads()}
     log("step %.2f ms (moments %.2f ms, assign %.2f ms, host eig %.2f ms, rechecked %d)"

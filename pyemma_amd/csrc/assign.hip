@@ -251,6 +251,29 @@ __global__ void k_zero_i64n(int64_t *p, int n) {
 // frames; lane l holds frame l % 32 and centers 8(e>>2) + 4(l>>5) + (e&3).
 typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
+// vmcnt(n): wait until at most n vector-memory operations of this wave are
+// outstanding (n a small run-time value; the instruction needs a constant)
+__device__ __forceinline__ void wait_vmcnt(int n) {
+    switch (n) {
+        case 0: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+        case 1: __builtin_amdgcn_s_waitcnt(0x0F71); break;
+        case 2: __builtin_amdgcn_s_waitcnt(0x0F72); break;
+        case 3: __builtin_amdgcn_s_waitcnt(0x0F73); break;
+        case 4: __builtin_amdgcn_s_waitcnt(0x0F74); break;
+        case 5: __builtin_amdgcn_s_waitcnt(0x0F75); break;
+        case 6: __builtin_amdgcn_s_waitcnt(0x0F76); break;
+        case 7: __builtin_amdgcn_s_waitcnt(0x0F77); break;
+        case 8: __builtin_amdgcn_s_waitcnt(0x0F78); break;
+        case 9: __builtin_amdgcn_s_waitcnt(0x0F79); break;
+        case 10: __builtin_amdgcn_s_waitcnt(0x0F7A); break;
+        case 11: __builtin_amdgcn_s_waitcnt(0x0F7B); break;
+        case 12: __builtin_amdgcn_s_waitcnt(0x0F7C); break;
+        case 13: __builtin_amdgcn_s_waitcnt(0x0F7D); break;
+        case 14: __builtin_amdgcn_s_waitcnt(0x0F7E); break;
+        default: __builtin_amdgcn_s_waitcnt(0x0F70); break;  // larger counts: wait for everything
+    }
+}
+
 template <int KSTEPS, int NW, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float CE,
@@ -272,8 +295,12 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // t - 1 run together and read the same buffer, while the copies fill the
     // other one.  Two separate objects, so that the compiler can tell the
     // buffer being read from the one being filled.
+    // three buffers (tiles prefetched two ahead) where they fit next to the
+    // occupancy of the variant, otherwise two
+    constexpr int NB = (3 * IMG * MINB <= 160 * 1024) ? 3 : 2;
     __shared__ __align__(16) uint8_t s_img0[IMG];
     __shared__ __align__(16) uint8_t s_img1[IMG];
+    __shared__ __align__(16) uint8_t s_img2[NB == 3 ? IMG : 16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -383,6 +410,46 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         }
     };
 
+    if constexpr (NB == 3) {
+        // Tiles prefetched two ahead: buffer u % 3 holds hi + lo of tile u and
+        // the constants of tile u - 1.  Iteration t waits until the copies of
+        // tile t are complete (those of tile t + 1 may still be in flight: the
+        // count of this wave's copies per fetch is known), synchronizes, starts
+        // the copies of tile t + 2 into the buffer read in iteration t - 1, and
+        // runs the MFMAs of tile t with the epilogue of tile t - 1.
+        int rw = 0;  // hi + lo copy instructions this wave issues per tile
+#pragma unroll
+        for (int i = 0; i < ROUNDS; ++i) rw += (i * NT + wave * 64 < MCH) ? 1 : 0;
+        auto copies = [&](int u) -> int {  // copy instructions of this wave in fetch(u)
+            return (u < ntiles ? rw : 0) + ((wave == 0 && u >= 1 && u - 1 < ntiles) ? 1 : 0);
+        };
+        fetch(ntiles > 1 ? 1 : -1, 0, s_img1);
+        // tile 0: no epilogue yet
+        wait_vmcnt(copies(1));
+        __syncthreads();
+        fetch(2 < ntiles ? 2 : -1, 1 < ntiles ? 1 : -1, s_img2);
+        floatx16 acc = chain(s_img0);
+        auto iter = [&](int t, uint8_t *cur, uint8_t *tgt) __attribute__((always_inline)) {
+            wait_vmcnt(copies(t + 1));
+            __syncthreads();
+            fetch(t + 2 < ntiles ? t + 2 : -1, t + 1 < ntiles ? t + 1 : -1, tgt);
+            const floatx16 nacc = chain(cur);
+            epilogue(acc, cur, t - 1);
+            interleave();
+            acc = nacc;
+        };
+        for (int t = 1; t < ntiles; t += 3) {
+            iter(t, s_img1, s_img0);
+            if (t + 1 >= ntiles) break;
+            iter(t + 1, s_img2, s_img1);
+            if (t + 2 >= ntiles) break;
+            iter(t + 2, s_img0, s_img2);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        const int rb = ntiles % 3;
+        epilogue(acc, rb == 0 ? s_img0 : (rb == 1 ? s_img1 : s_img2), ntiles - 1);
+    } else {
     // tile 0, then tiles 1 .. ntiles - 1 with the epilogue one tile behind.
     // Each barrier follows a wait for this wave's copies, so the buffer about
     // to be read is complete for every wave and every wave has finished
@@ -417,6 +484,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
     epilogue(acc, (ntiles & 1) ? s_img1 : s_img0, ntiles - 1);
+    }
 
     // center index of the element code: tile t, element e of the 32x32 output
     // (lane l holds centers 8 (e >> 2) + 4 (l >> 5) + (e & 3))
