@@ -223,6 +223,30 @@ int emma_row_affine(const float *X, int64_t n, int32_t d, const double *u, doubl
  */
 int emma_count_nonfinite(const float *X, int64_t n, int64_t *count, void *stream);
 
+/*
+ * Transition counts of discrete trajectories: C[i * nstates + j] (device
+ * int64, nstates^2 entries, overwritten) = number of t with dtraj[t] = i and
+ * dtraj[t + lag] = j within one trajectory, for every t (sliding != 0) or for
+ * t = 0, lag, 2 lag, ... (sliding == 0); hist (device int64, nstates entries,
+ * may be NULL) = frames per state.  Replaces deeptime's
+ * count_matrix(dtrajs, lag, sliding, nstates) called at
+ * msm/estimators/_dtraj_stats.py:138-145 (and count_states, :60) on the
+ * int32 labels that assignment produces (coordinates/clustering/interface.py:
+ * 157-231).  Frames with state -1 (unassigned) are not counted; the number of
+ * entries below -1 or at/above nstates goes to n_bad (device int64) and those
+ * entries are not counted.  traj_offsets: ntraj + 1 device int64 CSR offsets
+ * into dtraj (traj_offsets[ntraj] == n).
+ */
+int emma_count_transitions(const int32_t *dtraj, int64_t n, const int64_t *traj_offsets, int32_t ntraj,
+                           int32_t lag, int32_t sliding, int32_t nstates, int64_t *C, int64_t *hist,
+                           int64_t *n_bad, void *stream);
+
+/*
+ * out[0] (device int32) = largest entry of dtraj, -1 if n == 0: the state
+ * count of deeptime's number_of_states (msm/estimators/_dtraj_stats.py:64).
+ */
+int emma_max_state(const int32_t *dtraj, int64_t n, int32_t *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,9 @@ moments   FP64 NumPy restatement of the reference's lagged, mean-free moment
 eig       FP64 generalized eigenproblem C_t r = C_0 r l solved directly with
           LAPACK, the closed form that the reference's eig_corr
           (pyemma/_ext/variational/solvers/direct.py:225-295) approximates.
+counts    transition count matrix and state histogram of discrete
+          trajectories (deeptime's count_matrix / count_states as called at
+          pyemma/msm/estimators/_dtraj_stats.py:60-64, 138-145).
 native    ctypes binding of emma_oracle.c: the sequential FP32 assign loop,
           kmeans++ with a documented RNG/summation order, and Lloyd.
 

@@ -76,6 +76,10 @@ def lib():
     L.emma_project.argtypes = [vp, i64, i32, vp, vp, i32, i32, vp, vp]
     L.emma_count_nonfinite.restype = ctypes.c_int
     L.emma_count_nonfinite.argtypes = [vp, i64, vp, vp]
+    L.emma_count_transitions.restype = ctypes.c_int
+    L.emma_count_transitions.argtypes = [vp, i64, vp, i32, i32, i32, i32, vp, vp, vp, vp]
+    L.emma_max_state.restype = ctypes.c_int
+    L.emma_max_state.argtypes = [vp, i64, vp, vp]
     _lib = L
     return L
 
@@ -259,3 +263,27 @@ def row_affine(Xt, u, u0):
     out = t.empty(n, dtype=t.float64, device=Xt.device)
     check(lib().emma_row_affine(ptr(Xt), n, d, ptr(u_dev), float(u0), ptr(out), stream_ptr()), "emma_row_affine")
     return out
+
+
+def max_state(S_dev):
+    """Largest entry of an int32 device tensor (-1 when empty), as a device int32 tensor."""
+    t = torch()
+    out = t.empty(1, dtype=t.int32, device=S_dev.device)
+    check(lib().emma_max_state(ptr(S_dev), S_dev.numel(), ptr(out), stream_ptr()), "emma_max_state")
+    return out
+
+
+def count_transitions(S_dev, traj_offsets, lag, sliding, nstates, hist=True):
+    """Dense transition counts (nstates x nstates int64), state histogram and the
+    number of out-of-range entries, all device tensors."""
+    t = torch()
+    off = traj_offsets
+    if not isinstance(off, t.Tensor):
+        off = t.as_tensor(np.asarray(off, dtype=np.int64), device=S_dev.device)
+    C = t.empty((nstates, nstates), dtype=t.int64, device=S_dev.device)
+    H = t.empty(nstates, dtype=t.int64, device=S_dev.device) if hist else None
+    bad = t.empty(1, dtype=t.int64, device=S_dev.device)
+    check(lib().emma_count_transitions(ptr(S_dev), S_dev.numel(), ptr(off), off.numel() - 1, int(lag),
+                                       1 if sliding else 0, int(nstates), ptr(C), ptr(H), ptr(bad), stream_ptr()),
+          "emma_count_transitions")
+    return C, H, bad

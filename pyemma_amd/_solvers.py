@@ -21,7 +21,7 @@ try:  # bounds the BLAS threads of the small d x d problems
 except ImportError:  # pragma: no cover - optional
     threadpool_limits = None
 
-__all__ = ["sort_by_norm", "spd_eig", "spd_inv_split", "eig_corr", "blas_threads"]
+__all__ = ["sort_by_norm", "spd_eig", "spd_inv_split", "spd_inv_sqrt", "eig_corr", "blas_threads"]
 
 # The d x d eigenproblems (d <= a few hundred) run fastest on a few BLAS
 # threads: with one thread per core OpenBLAS spends most of the time in
@@ -91,6 +91,22 @@ def spd_inv_split(W, epsilon=1e-10, method="QR", canonical_signs=False):
         return 1.0 / np.sqrt(W[0, 0])
     sm, Vm = spd_eig(W, epsilon=epsilon, method=method, canonical_signs=canonical_signs)
     return np.dot(Vm, np.diag(1.0 / np.sqrt(sm)))
+
+
+def spd_inv_sqrt(W, epsilon=1e-10, method="QR", return_rank=False):
+    """direct.py:142-181: W^-1/2 = V diag(s^-1/2) V^T after rank truncation."""
+    if np.shape(W)[0] == 1:
+        if W[0, 0] < epsilon:
+            raise ZeroRankError("All eigenvalues are smaller than %g, rank reduction would discard all dimensions."
+                                % epsilon)
+        Winv = 1.0 / np.sqrt(W[0, 0])
+        sm = np.ones(1)
+    else:
+        sm, Vm = spd_eig(W, epsilon=epsilon, method=method)
+        Winv = np.dot(Vm, np.diag(1.0 / np.sqrt(sm))).dot(Vm.T)
+    if return_rank:
+        return Winv, sm.shape[0]
+    return Winv
 
 
 def eig_corr(C0, Ct, epsilon=1e-10, method="QR", sign_maxelement=False):

@@ -1,8 +1,10 @@
 """pyemma.coordinates-compatible entry points of the GPU discretization path."""
-from .api import assign_to_centers, cluster_kmeans, covariance_lagged, tica
+from .api import assign_to_centers, cluster_kmeans, covariance_lagged, pca, tica, vamp
 from .clustering import AssignCenters, KmeansClustering
 from .covariance import LaggedCovariance
+from .pca import PCA
 from .tica import TICA
+from .vamp import VAMP
 
-__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "TICA", "LaggedCovariance",
-           "KmeansClustering", "AssignCenters"]
+__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "vamp", "pca", "TICA",
+           "LaggedCovariance", "KmeansClustering", "AssignCenters", "VAMP", "PCA"]

@@ -4,6 +4,8 @@ tica             api.py:1046-1290
 covariance_lagged api.py:1554-1661
 cluster_kmeans   api.py:1727-1866
 assign_to_centers api.py:2060-2156
+vamp             api.py:1294-1443
+pca              api.py:900-1042
 """
 import types
 import warnings
@@ -12,9 +14,11 @@ import numpy as np
 
 from .clustering import AssignCenters, KmeansClustering
 from .covariance import LaggedCovariance
+from .pca import PCA
 from .tica import TICA
+from .vamp import VAMP
 
-__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers"]
+__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "vamp", "pca"]
 
 
 def _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=True, for_tica=True, precision="fp64"):
@@ -60,6 +64,31 @@ def tica(data=None, lag=10, dim=-1, var_cutoff=0.95, kinetic_map=True, commute_m
                       "it cannot be guaranteed that all eigenvalues will be smaller than one.", DeprecationWarning)
     res = TICA(lag, dim=dim, var_cutoff=var_cutoff, kinetic_map=kinetic_map, commute_map=commute_map, skip=skip,
                stride=stride, weights=weights, reversible=reversible, ncov_max=ncov_max, precision=precision)
+    if data is not None:
+        res.estimate(data, chunksize=chunksize)
+    else:
+        res.chunksize = chunksize
+    return res
+
+
+def vamp(data=None, lag=10, dim=None, scaling=None, right=False, ncov_max=float("inf"), stride=1, skip=0,
+         chunksize=None, **kwargs):
+    r"""Variational approach for Markov processes (api.py:1294-1443)."""
+    res = VAMP(lag, dim=dim, scaling=scaling, right=right, skip=skip, ncov_max=ncov_max,
+               precision=kwargs.pop("precision", "fp64"))
+    if data is not None:
+        res.estimate(data, stride=stride, chunksize=chunksize)
+    else:
+        res.chunksize = chunksize
+    return res
+
+
+def pca(data=None, dim=-1, var_cutoff=0.95, stride=1, mean=None, skip=0, chunksize=None, **kwargs):
+    r"""Principal component analysis (api.py:900-1042)."""
+    if mean is not None:
+        warnings.warn("provided mean ignored", DeprecationWarning)
+    res = PCA(dim=dim, var_cutoff=var_cutoff, mean=None, skip=skip, stride=stride,
+              precision=kwargs.pop("precision", "fp64"))
     if data is not None:
         res.estimate(data, chunksize=chunksize)
     else:

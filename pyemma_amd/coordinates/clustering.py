@@ -103,6 +103,13 @@ class AbstractClustering(object):
         self.n_rechecked = int(nre.item())
         return labels
 
+    def assign_device(self, X):
+        """Labels of X kept on the device (DeviceDtrajs), the input of pyemma_amd.dtraj's counts."""
+        from ..dtraj import DeviceDtrajs
+
+        dt = to_device(X)
+        return DeviceDtrajs(self._assign_device(dt), dt.offsets)
+
     def transform(self, X):
         """(T, 1) int32 label columns (interface.py:157-167)."""
         single = not isinstance(X, (list, tuple, DeviceTrajectories))

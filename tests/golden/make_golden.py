@@ -11,6 +11,13 @@ hmm_small.npz
     generator) with the FP64 oracle's TICA moments and eigenvalues at lag 10,
     the oracle's kmeans++ picks (k=16, seed 42), 5 Lloyd iterations from them
     and the sequential-FP32 labels.  These pin the oracle between rounds.
+oom_counts.npz
+    The discrete trajectories and the reference count tensor of the
+    reference's OOM-MSM tests (pyemma/msm/tests/data/TestData_OOM_MSM.npz,
+    read with allow_pickle=False; used at msm/tests/test_oom_msm.py:94-120):
+    1000 trajectories of 200 frames over 5 states (uint8, in key order
+    arr_0 .. arr_999) and C2t (5, 5, 5).  C2t summed over its last axis is
+    the sliding-window count matrix at lag 10 of those trajectories.
 
 Run from the repository root: python tests/golden/make_golden.py
 (the Koopman file is only re-read when /root/reference is present).
@@ -30,6 +37,7 @@ from oracle import native as on  # noqa: E402
 from pyemma_amd.synthetic import hmm_trajectory  # noqa: E402
 
 REF_KOOPMAN = "/root/reference/pyemma/coordinates/tests/data/test_data_koopman.npz"
+REF_OOM = "/root/reference/pyemma/msm/tests/data/TestData_OOM_MSM.npz"
 
 
 def make_koopman():
@@ -42,6 +50,16 @@ def make_koopman():
     assert states.shape == (1000, 200) and states.max() < 20
     np.savez_compressed(os.path.join(HERE, "koopman_states.npz"), states=states,
                         keys=np.array(keys))
+
+
+def make_oom_counts():
+    if not os.path.exists(REF_OOM):
+        print("reference fixture not present; keeping", os.path.join(HERE, "oom_counts.npz"))
+        return
+    f = np.load(REF_OOM, allow_pickle=False)
+    dtrajs = np.stack([f["arr_%d" % k] for k in range(1000)])
+    assert dtrajs.shape == (1000, 200) and dtrajs.min() >= 0 and dtrajs.max() < 5
+    np.savez_compressed(os.path.join(HERE, "oom_counts.npz"), dtrajs=dtrajs.astype(np.uint8), C2t=f["C2t"])
 
 
 def make_hmm():
@@ -59,5 +77,6 @@ def make_hmm():
 
 if __name__ == "__main__":
     make_koopman()
+    make_oom_counts()
     make_hmm()
     print("ok")

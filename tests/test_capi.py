@@ -52,6 +52,9 @@ def test_version_and_workspace_sizes():
     ("lloyd_mode", "unknown assign mode"),
     ("project_sizes", "bad sizes"),
     ("nonfinite_neg", "n < 0"),
+    ("counts_lag0", "bad sizes"),
+    ("counts_big", "too large"),
+    ("maxstate_null", "bad arguments"),
 ])
 def test_invalid_arguments_rejected_on_host(call, expect):
     L = _native.lib()
@@ -73,6 +76,12 @@ def test_invalid_arguments_rejected_on_host(call, expect):
         rc = L.emma_lloyd(p, 10, 3, p, 2, 5, 0.0, 9, p, p, p, p, p, 1 << 20, z)
     elif call == "project_sizes":
         rc = L.emma_project(p, 10, 0, p, p, 1, 1, p, z)
+    elif call == "counts_lag0":
+        rc = L.emma_count_transitions(p, 10, p, 1, 0, 1, 5, p, p, p, z)
+    elif call == "counts_big":
+        rc = L.emma_count_transitions(p, 10, p, 1, 1, 1, 50000, p, p, p, z)
+    elif call == "maxstate_null":
+        rc = L.emma_max_state(p, 10, z, z)
     else:
         rc = L.emma_count_nonfinite(p, -1, p, z)
     assert rc == -1
