@@ -247,6 +247,23 @@ int emma_count_transitions(const int32_t *dtraj, int64_t n, const int64_t *traj_
  */
 int emma_max_state(const int32_t *dtraj, int64_t n, int32_t *out, void *stream);
 
+#define EMMA_DIST_INVERSE 1u /* 1 / distance (InverseDistanceFeature) */
+
+/*
+ * out[f * n_pairs + p] (device float32) = distance between atoms pairs[2p]
+ * and pairs[2p + 1] of frame f of xyz (device float32, n_frames x n_atoms x
+ * 3, row-major): sqrt((dx*dx + dy*dy) + dz*dz) with d = x_j - x_i, each step
+ * a correctly rounded FP32 operation, or its reciprocal with
+ * EMMA_DIST_INVERSE.  Replaces DistanceFeature.transform /
+ * InverseDistanceFeature.transform (coordinates/data/featurization/
+ * distances.py:53-54, 73-74: mdtraj.compute_distances without a unit cell)
+ * behind Featurizer.add_distances / add_inverse_distances
+ * (featurizer.py:305-342).  Pairs (device int32, n_pairs x 2) with an index
+ * outside [0, n_atoms) give NaN.  n_atoms <= 4096.
+ */
+int emma_pair_distances(const float *xyz, int64_t n_frames, int32_t n_atoms, const int32_t *pairs, int32_t n_pairs,
+                        uint32_t flags, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,9 @@ eig       FP64 generalized eigenproblem C_t r = C_0 r l solved directly with
 counts    transition count matrix and state histogram of discrete
           trajectories (deeptime's count_matrix / count_states as called at
           pyemma/msm/estimators/_dtraj_stats.py:60-64, 138-145).
+featurize FP32 pairwise distances of particle coordinates (mdtraj's
+          non-periodic compute_distances as DistanceFeature.transform calls it,
+          coordinates/data/featurization/distances.py:53-54).
 native    ctypes binding of emma_oracle.c: the sequential FP32 assign loop,
           kmeans++ with a documented RNG/summation order, and Lloyd.
 

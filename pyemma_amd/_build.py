@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libemma_amd.so")
-SOURCES = ["common.hip", "moments.hip", "assign.hip", "kmeans.hip", "counts.hip"]
+SOURCES = ["common.hip", "moments.hip", "assign.hip", "kmeans.hip", "counts.hip", "featurize.hip"]
 
 # -ffp-contract=off: the reference distance loop must not be fused into FMAs;
 # kernels use fmaf()/fma() explicitly where a fused operation is intended.

@@ -24,6 +24,7 @@ EMMA_PREC_BF16X4 = 1
 EMMA_PREC_FP64 = 2
 EMMA_ASSIGN_FP32_PLAIN = 0
 EMMA_ASSIGN_BF16X3_RECHECK = 1
+EMMA_DIST_INVERSE = 1
 
 _lib = None
 
@@ -78,6 +79,8 @@ def lib():
     L.emma_count_nonfinite.argtypes = [vp, i64, vp, vp]
     L.emma_count_transitions.restype = ctypes.c_int
     L.emma_count_transitions.argtypes = [vp, i64, vp, i32, i32, i32, i32, vp, vp, vp, vp]
+    L.emma_pair_distances.restype = ctypes.c_int
+    L.emma_pair_distances.argtypes = [vp, i64, i32, vp, i32, u32, vp, vp]
     L.emma_max_state.restype = ctypes.c_int
     L.emma_max_state.argtypes = [vp, i64, vp, vp]
     _lib = L

@@ -2,9 +2,10 @@
 from .api import assign_to_centers, cluster_kmeans, covariance_lagged, pca, tica, vamp
 from .clustering import AssignCenters, KmeansClustering
 from .covariance import LaggedCovariance
+from .featurizer import Featurizer, featurizer
 from .pca import PCA
 from .tica import TICA
 from .vamp import VAMP
 
 __all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "vamp", "pca", "TICA",
-           "LaggedCovariance", "KmeansClustering", "AssignCenters", "VAMP", "PCA"]
+           "LaggedCovariance", "KmeansClustering", "AssignCenters", "VAMP", "PCA", "Featurizer", "featurizer"]

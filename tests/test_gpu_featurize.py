@@ -1,0 +1,77 @@
+"""Pairwise-distance features on the GPU (emma_pair_distances), bit-exact
+against the FP32 oracle (oracle/featurize.py): every step is a correctly
+rounded FP32 operation in the oracle's order."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle.featurize import pair_distances
+from pyemma_amd import _native
+from pyemma_amd.coordinates import api
+from pyemma_amd.coordinates.featurizer import Featurizer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n_atoms,n_frames", [(2, 1), (25, 10007), (25, 200000), (7, 3), (300, 513), (4096, 3)])
+def test_all_pairs_bit_exact(n_atoms, n_frames):
+    rng = np.random.RandomState(n_atoms)
+    xyz = (rng.randn(n_frames, n_atoms, 3) * 2.5 + 1.0).astype(np.float32)
+    f = Featurizer(n_atoms)
+    sel = np.arange(n_atoms) if n_atoms <= 300 else np.arange(0, n_atoms, 64)
+    f.add_distances(sel)
+    got = f.transform(xyz)
+    np.testing.assert_array_equal(got, pair_distances(xyz, f.active_features[0].distance_indexes))
+
+
+def test_inverse_and_concatenation():
+    rng = np.random.RandomState(1)
+    xyz = rng.randn(4000, 10, 3).astype(np.float32)
+    f = Featurizer(10)
+    f.add_distances([[0, 1], [2, 9], [9, 2]])
+    f.add_inverse_distances([0, 1, 2], indices2=[3, 4, 7])
+    got = f.transform(xyz)
+    ref = np.hstack([pair_distances(xyz, a.distance_indexes, inverse=a.inverse) for a in f.active_features])
+    assert got.shape == (4000, f.dimension())
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_exact_integer_distances():
+    quads = [(1, 2, 2, 3), (2, 3, 6, 7), (1, 4, 8, 9), (4, 4, 7, 9), (3, 4, 12, 13)]
+    xyz = np.zeros((len(quads), 2, 3), dtype=np.float32)
+    for k, (a, b, c, _) in enumerate(quads):
+        xyz[k, 1] = (a, b, c)
+    f = Featurizer(2)
+    f.add_distances([[0, 1]])
+    np.testing.assert_array_equal(f.transform(xyz)[:, 0], [q[3] for q in quads])
+
+
+def test_out_of_range_pair_gives_nan():
+    """The C ABI does not trust device pair indices: out-of-range pairs read nothing and give NaN."""
+    t = _native.torch()
+    xyz = t.randn((100, 4, 3), device=_native.device())
+    pairs = t.tensor([[0, 1], [2, 4], [-1, 0], [3, 2]], dtype=t.int32, device=xyz.device)
+    out = t.empty((100, 4), device=xyz.device)
+    _native.check(_native.lib().emma_pair_distances(_native.ptr(xyz), 100, 4, _native.ptr(pairs), 4, 0,
+                                                    _native.ptr(out), _native.stream_ptr()), "emma_pair_distances")
+    o = out.cpu().numpy()
+    assert np.all(np.isnan(o[:, 1])) and np.all(np.isnan(o[:, 2]))
+    np.testing.assert_array_equal(o[:, [0, 3]], pair_distances(xyz.cpu().numpy(), [[0, 1], [3, 2]]))
+
+
+def test_features_feed_tica_and_assign():
+    """Featurize on the device, then TICA and assignment on the device-resident features."""
+    rng = np.random.RandomState(4)
+    trajs = [np.cumsum(rng.randn(n, 8, 3) * 0.05, axis=0).astype(np.float32) for n in (3000, 2500)]
+    f = Featurizer(8)
+    f.add_distances(np.arange(8))
+    dt = f.transform_trajectories(trajs)
+    host = [pair_distances(x, f.active_features[0].distance_indexes) for x in trajs]
+    tic_dev = api.tica(dt, lag=5, precision="fp64")
+    tic_host = api.tica(host, lag=5, precision="fp64")
+    np.testing.assert_array_equal(tic_dev.cov, tic_host.cov)
+    np.testing.assert_array_equal(tic_dev.eigenvalues, tic_host.eigenvalues)
+    centers = host[0][::300]
+    np.testing.assert_array_equal(np.concatenate(api.assign_to_centers(dt, centers)),
+                                  np.concatenate(api.assign_to_centers(host, centers)))
