@@ -5,6 +5,7 @@ covariance_lagged api.py:1554-1661
 cluster_kmeans   api.py:1727-1866
 assign_to_centers api.py:2060-2156
 vamp             api.py:1294-1443
+tica_nystroem    api.py:1446-1551
 pca              api.py:900-1042
 """
 import types
@@ -14,11 +15,12 @@ import numpy as np
 
 from .clustering import AssignCenters, KmeansClustering
 from .covariance import LaggedCovariance
+from .nystroem import NystroemTICA
 from .pca import PCA
 from .tica import TICA
 from .vamp import VAMP
 
-__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "vamp", "pca"]
+__all__ = ["tica", "covariance_lagged", "cluster_kmeans", "assign_to_centers", "vamp", "pca", "tica_nystroem"]
 
 
 def _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=True, for_tica=True, precision="fp64"):
@@ -76,6 +78,20 @@ def vamp(data=None, lag=10, dim=None, scaling=None, right=False, ncov_max=float(
     r"""Variational approach for Markov processes (api.py:1294-1443)."""
     res = VAMP(lag, dim=dim, scaling=scaling, right=right, skip=skip, ncov_max=ncov_max,
                precision=kwargs.pop("precision", "fp64"))
+    if data is not None:
+        res.estimate(data, stride=stride, chunksize=chunksize)
+    else:
+        res.chunksize = chunksize
+    return res
+
+
+def tica_nystroem(max_columns, data=None, lag=10, dim=-1, var_cutoff=0.95, epsilon=1e-6, stride=1, skip=0,
+                  reversible=True, ncov_max=float("inf"), chunksize=None, initial_columns=None, nsel=1, neig=None,
+                  **kwargs):
+    r"""Sparse-sampling (Nystroem) TICA (api.py:1446-1551)."""
+    res = NystroemTICA(lag, max_columns, dim=dim, var_cutoff=var_cutoff, epsilon=epsilon, skip=skip,
+                       reversible=reversible, ncov_max=ncov_max, initial_columns=initial_columns, nsel=nsel,
+                       neig=neig, precision=kwargs.pop("precision", "fp64"))
     if data is not None:
         res.estimate(data, stride=stride, chunksize=chunksize)
     else:
