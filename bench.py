@@ -83,7 +83,7 @@ def main():
     dev = _native.device()
     default_workload = (args.ntraj, args.nframes, args.k, args.lag) == (200, 50000, 1000, 20)
     t0 = time.time()
-    X, offsets = brownian_distances(args.ntraj, args.nframes, seed=2, device=dev)
+    X, offsets, xyz = brownian_distances(args.ntraj, args.nframes, seed=2, device=dev, return_positions=True)
     torch.cuda.synchronize()
     N, d = X.shape
     log("data %d x %d generated in %.1f s" % (N, d, time.time() - t0))
@@ -173,6 +173,9 @@ def main():
                             "tflops": f_mom / (fp64_ms * 1e-3) / 1e12}
     log("fp64 moments %.2f ms" % fp64_ms)
 
+    kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
+    del xyz
+
     plain = None
     if not args.no_plain:
         t1 = time.perf_counter()
@@ -206,6 +209,60 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def _timed(fn, reps=3):
+    """Average device time of fn() over reps calls after one warm-up, HIP events on the current stream."""
+    import torch
+
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def adjacent_legs(args, X, off_dev, offsets, labels, xyz):
+    """The producer and consumers either side of the path, on the same workload:
+    pairwise-distance featurization of the particle positions (HBM-bound),
+    transition counts of the assigned labels at the TICA lag (HBM/atomic-bound)
+    and the VAMP covariances C00, C0t, Ctt (non-reversible moments)."""
+    import torch
+
+    from pyemma_amd import _native
+    from pyemma_amd.coordinates.featurizer import Featurizer
+
+    N, d = X.shape
+    out = {}
+    na = xyz.shape[1]
+    f = Featurizer(na)
+    f.add_distances(np.arange(na))
+    Y = f.transform_device(xyz)
+    diff = float((Y - X).abs().max().item())
+    ms = _timed(lambda: f.transform_device(xyz))
+    nbytes = N * (12 * na + 4 * d)
+    out["featurize"] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                        "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "max_abs_diff_vs_fp64_features": diff}
+    del Y
+    ms = _timed(lambda: _native.count_transitions(labels, off_dev, args.lag, True, args.k))
+    C, _, bad = _native.count_transitions(labels, off_dev, args.lag, True, args.k)
+    total = int(C.sum().item())
+    expect = int(sum(max(0, (b - a) - args.lag) for a, b in zip(offsets[:-1], offsets[1:])))
+    nbytes = 4 * N + 8 * args.k * args.k
+    out["counts"] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                     "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     "pairs_counted": total, "pairs_expected": expect, "bad": int(bad.item())}
+    ms = _timed(lambda: _native.moments_lagged(X, off_dev, args.lag, reversible=False, remove_mean=True, c0t=True,
+                                               ctt=True, precision=_native.EMMA_PREC_BF16X4), reps=1)
+    out["vamp_moments"] = {"ms": ms, "precision": "bf16x4", "matrices": "C00, C0t, Ctt (separate means)"}
+    log("featurize %.2f ms, counts %.2f ms, vamp moments %.2f ms"
+        % (out["featurize"]["ms"], out["counts"]["ms"], out["vamp_moments"]["ms"]))
+    torch.cuda.synchronize()
+    return out
 
 
 def serving_leg(args):

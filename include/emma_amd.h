@@ -259,7 +259,7 @@ int emma_max_state(const int32_t *dtraj, int64_t n, int32_t *out, void *stream);
  * distances.py:53-54, 73-74: mdtraj.compute_distances without a unit cell)
  * behind Featurizer.add_distances / add_inverse_distances
  * (featurizer.py:305-342).  Pairs (device int32, n_pairs x 2) with an index
- * outside [0, n_atoms) give NaN.  n_atoms <= 4096.
+ * outside [0, n_atoms) give NaN.  n_atoms <= 2730.
  */
 int emma_pair_distances(const float *xyz, int64_t n_frames, int32_t n_atoms, const int32_t *pairs, int32_t n_pairs,
                         uint32_t flags, float *out, void *stream);

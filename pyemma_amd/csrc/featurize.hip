@@ -17,40 +17,64 @@ namespace emma {
 namespace feat {
 
 constexpr int NT = 256;
-constexpr int LDS_FLOATS = 12288;  // 48 KiB of staged coordinates per workgroup
+constexpr int LDS_FLOATS = 8192;   // 32 KiB of staged coordinates per workgroup
+constexpr int PAIR_CHUNK = 4096;   // pairs staged per pass (16 KiB)
+constexpr uint32_t BAD_PAIR = 0xFFFFFFFFu;
 
+// One workgroup: a block of frames staged in LDS, then the pairs in chunks
+// of PAIR_CHUNK, each pair packed as the LDS offsets (3 i) | (3 j) << 16.  The
+// element loop walks (frame, pair) with incremental counters, so it has no
+// division and no global loads: LDS reads, VALU and one coalesced store.
 __global__ __launch_bounds__(NT) void k_pair_distances(const float *__restrict__ xyz, int64_t n_frames,
                                                        int n_atoms, const int32_t *__restrict__ pairs,
                                                        int n_pairs, int frames_per_block, int inverse,
                                                        float *__restrict__ out) {
     __shared__ float s_xyz[LDS_FLOATS];
+    __shared__ uint32_t s_pair[PAIR_CHUNK];
+    const int tid = threadIdx.x;
     const int64_t f0 = (int64_t)blockIdx.x * frames_per_block;
     const int nf = (int)((n_frames - f0) < frames_per_block ? (n_frames - f0) : frames_per_block);
     const int per_frame = 3 * n_atoms;
     const int nval = nf * per_frame;  // <= LDS_FLOATS by the host's choice of frames_per_block
     const float *src = xyz + f0 * per_frame;
-    for (int e = threadIdx.x; e < nval; e += NT) s_xyz[e] = src[e];
-    __syncthreads();
-    const int nout = nf * n_pairs;  // < 2^31 by the host's choice of frames_per_block
+    for (int e = tid; e < nval; e += NT) s_xyz[e] = src[e];
     float *dst = out + f0 * n_pairs;
-    for (int e = threadIdx.x; e < nout; e += NT) {
-        const int f = e / n_pairs;
-        const int p = e - f * n_pairs;
-        const int i = pairs[2 * p], j = pairs[2 * p + 1];
-        float r;
-        if (i < 0 || i >= n_atoms || j < 0 || j >= n_atoms) {
-            r = __int_as_float(0x7fc00000);
-        } else {
-            const float *a = s_xyz + f * per_frame + 3 * i;
-            const float *b = s_xyz + f * per_frame + 3 * j;
-            const float dx = __fsub_rn(b[0], a[0]);
-            const float dy = __fsub_rn(b[1], a[1]);
-            const float dz = __fsub_rn(b[2], a[2]);
-            const float s = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-            r = sqrtf(s);  // correctly rounded expansion (approximate root + one-ulp fma correction)
-            if (inverse) r = __fdiv_rn(1.0f, r);
+    for (int pc = 0; pc < n_pairs; pc += PAIR_CHUNK) {
+        const int np = (n_pairs - pc) < PAIR_CHUNK ? (n_pairs - pc) : PAIR_CHUNK;
+        __syncthreads();  // coordinates staged; the previous chunk's pairs no longer read
+        for (int q = tid; q < np; q += NT) {
+            const int i = pairs[2 * (pc + q)], j = pairs[2 * (pc + q) + 1];
+            const bool ok = i >= 0 && i < n_atoms && j >= 0 && j < n_atoms;
+            s_pair[q] = ok ? (uint32_t)(3 * i) | ((uint32_t)(3 * j) << 16) : BAD_PAIR;
         }
-        dst[e] = r;
+        __syncthreads();
+        const int total = nf * np;  // < 2^31 by the host's choice of frames_per_block
+        const int df = NT / np, dp = NT - df * np;
+        int f = tid / np, p = tid - f * np;
+        for (int e = tid; e < total; e += NT) {
+            const uint32_t pk = s_pair[p];
+            float r;
+            if (pk == BAD_PAIR) {
+                r = __int_as_float(0x7fc00000);
+            } else {
+                const float *fr = s_xyz + f * per_frame;
+                const float *a = fr + (pk & 0xFFFFu);
+                const float *b = fr + (pk >> 16);
+                const float dx = __fsub_rn(b[0], a[0]);
+                const float dy = __fsub_rn(b[1], a[1]);
+                const float dz = __fsub_rn(b[2], a[2]);
+                const float s2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                r = sqrtf(s2);  // correctly rounded expansion (approximate root + one-ulp fma correction)
+                if (inverse) r = __fdiv_rn(1.0f, r);
+            }
+            dst[(int64_t)f * n_pairs + pc + p] = r;
+            p += dp;
+            f += df;
+            if (p >= np) {
+                p -= np;
+                ++f;
+            }
+        }
     }
 }
 
@@ -74,7 +98,7 @@ extern "C" int emma_pair_distances(const float *xyz, int64_t n_frames, int32_t n
     int fpb = LDS_FLOATS / (3 * n_atoms);
     const int64_t want = cdiv(n_frames, 1024);
     if (want < fpb) fpb = (int)(want < 1 ? 1 : want);
-    if ((int64_t)fpb * n_pairs > INT32_MAX - NT) fpb = (int)((INT32_MAX - NT) / n_pairs);
+    if ((int64_t)fpb * PAIR_CHUNK > INT32_MAX - NT) fpb = (int)((INT32_MAX - NT) / PAIR_CHUNK);
     if (fpb < 1) fpb = 1;
     hipStream_t s = (hipStream_t)stream;
     k_pair_distances<<<(unsigned)cdiv(n_frames, fpb), NT, 0, s>>>(xyz, n_frames, n_atoms, pairs, n_pairs, fpb,

@@ -36,11 +36,12 @@ def hmm_trajectory(n, d, seed=1, nstates=4, p_stay=0.99, mean_scale=5.0):
 
 
 def brownian_distances(ntraj, nframes, nparticles=25, sigma=0.05, kappa_dt=0.01, burn_in=1000, seed=2,
-                       device=None, basin_scale=1.5):
+                       device=None, basin_scale=1.5, return_positions=False):
     """Pairwise distances (i < j) of particles doing Euler-Maruyama Brownian motion
     under U(r) = kappa/2 min_b |r - R_b|^2 (3 basins R_b ~ N(0, basin_scale^2 I)),
     step r <- r - kappa_dt (r - R_b*) + sigma xi.  Distances in FP64, stored as
-    float32.  Returns (X (ntraj*nframes, P) tensor, offsets ndarray)."""
+    float32.  Returns (X (ntraj*nframes, P) tensor, offsets ndarray), and with
+    return_positions the float32 positions (ntraj*nframes, nparticles, 3) as well."""
     t = _torch()
     dev = device or ("cuda" if t.cuda.is_available() else "cpu")
     g = t.Generator(device=dev).manual_seed(seed)
@@ -52,6 +53,7 @@ def brownian_distances(ntraj, nframes, nparticles=25, sigma=0.05, kappa_dt=0.01,
     ju = t.as_tensor(ju, device=dev)
     P = len(iu)
     out = t.empty((ntraj, nframes, P), dtype=t.float32, device=dev)
+    xyz = t.empty((ntraj, nframes, nparticles, 3), dtype=t.float32, device=dev) if return_positions else None
 
     def step(r, noise):
         diff = r[:, :, None, :] - R[None, None, :, :]          # (ntraj, np, 3, 3)
@@ -74,7 +76,11 @@ def brownian_distances(ntraj, nframes, nparticles=25, sigma=0.05, kappa_dt=0.01,
         p = pos[:nb]
         dist = (p[:, :, iu, :] - p[:, :, ju, :]).norm(dim=-1)  # (nb, ntraj, P)
         out[:, t0:t0 + nb, :] = dist.permute(1, 0, 2).to(t.float32)
+        if xyz is not None:
+            xyz[:, t0:t0 + nb] = p.permute(1, 0, 2, 3).to(t.float32)
     offsets = np.arange(ntraj + 1, dtype=np.int64) * nframes
+    if return_positions:
+        return out.reshape(ntraj * nframes, P), offsets, xyz.reshape(ntraj * nframes, nparticles, 3)
     return out.reshape(ntraj * nframes, P), offsets
 
 

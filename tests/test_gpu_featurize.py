@@ -14,7 +14,7 @@ from pyemma_amd.coordinates.featurizer import Featurizer
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n_atoms,n_frames", [(2, 1), (25, 10007), (25, 200000), (7, 3), (300, 513), (4096, 3)])
+@pytest.mark.parametrize("n_atoms,n_frames", [(2, 1), (25, 10007), (25, 200000), (7, 3), (300, 513), (2730, 3)])
 def test_all_pairs_bit_exact(n_atoms, n_frames):
     rng = np.random.RandomState(n_atoms)
     xyz = (rng.randn(n_frames, n_atoms, 3) * 2.5 + 1.0).astype(np.float32)
