@@ -595,34 +595,10 @@ struct PairCursor {
 };
 
 template <bool DIAG>
-__global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
+__device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&s_sum)[4][RTB], int I, int J,
+                                         int task, int slice) {
     constexpr int PPT = DIAG ? 4 : 8;  // pairs per loader thread per stage
     constexpr int TB = RTB;
-    __shared__ __align__(16) RevLds lds;
-    __shared__ double s_sum[4][TB];
-
-    // task of this launch and XCD-aware numbering as in k_main
-    const int n_launch_tasks = DIAG ? p.nb : p.n_sym - p.nb;
-    const int per_xcd = gridDim.x >> 3;
-    const int wg = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (wg >= n_launch_tasks * p.n_slices) return;
-    const int lt = wg % n_launch_tasks;
-    const int slice = wg / n_launch_tasks;
-    int I, J;
-    if (DIAG) {
-        I = J = lt;
-    } else {
-        int i = 0, rem = lt;  // lt-th pair I < J in row-major order
-        while (rem >= p.nb - 1 - i) {
-            rem -= p.nb - 1 - i;
-            ++i;
-        }
-        I = i;
-        J = i + 1 + rem;
-    }
-    int task = 0;
-    for (int i = 0; i < I; ++i) task += p.nb - i;
-    task += J - I;
 
     const int64_t P = *p.P_total;
     const int64_t per = (P + p.n_slices - 1) / p.n_slices;
@@ -906,6 +882,41 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     }
 }
 
+// All tasks of the reversible bf16 moments in one launch.  The workgroups of
+// one pair slice are consecutive in one XCD's dispatch order (off-diagonal
+// tasks first, then the diagonal ones), so the tasks that read the same frame
+// rows run side by side and share the XCD's L2: each frame comes from HBM
+// about once per call instead of once per kind of task.
+__global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
+    __shared__ __align__(16) RevLds lds;
+    __shared__ double s_sum[4][RTB];
+    const int per_xcd = gridDim.x >> 3;
+    const int wg = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (wg >= p.n_sym * p.n_slices) return;
+    const int slice = wg / p.n_sym;
+    const int lt = wg - slice * p.n_sym;
+    const int n_off = p.n_sym - p.nb;
+    int I, J;
+    if (lt >= n_off) {
+        I = J = lt - n_off;
+    } else {
+        int i = 0, rem = lt;  // lt-th pair I < J in row-major order
+        while (rem >= p.nb - 1 - i) {
+            rem -= p.nb - 1 - i;
+            ++i;
+        }
+        I = i;
+        J = i + 1 + rem;
+    }
+    int task = 0;
+    for (int i = 0; i < I; ++i) task += p.nb - i;
+    task += J - I;
+    if (I == J)
+        rev_body<true>(p, lds, s_sum, I, J, task, slice);
+    else
+        rev_body<false>(p, lds, s_sum, I, J, task, slice);
+}
+
 // ------------------------------------------------------------------
 // frame weights: sum of the weights of every trajectory's pairs, then the
 // total (both in a fixed order)
@@ -1057,12 +1068,7 @@ static void launch_kind_w(const Params &p, hipStream_t s) {
     const unsigned g_sym = (unsigned)(cdiv((int64_t)n_sym * p.n_slices, 8) * 8);
     if constexpr (PREC == EMMA_PREC_BF16X4 && !WT) {
         if (p.rev) {
-            const unsigned g_diag = (unsigned)(cdiv((int64_t)p.nb * p.n_slices, 8) * 8);
-            k_rev_bf16<true><<<g_diag, RNT, 0, s>>>(p);
-            if (n_sym > p.nb) {
-                const unsigned g_off = (unsigned)(cdiv((int64_t)(n_sym - p.nb) * p.n_slices, 8) * 8);
-                k_rev_bf16<false><<<g_off, RNT, 0, s>>>(p);
-            }
+            k_rev_bf16<<<g_sym, RNT, 0, s>>>(p);
             return;
         }
     }
@@ -1127,8 +1133,8 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
     const bool weighted = (flags & EMMA_MOM_WEIGHTED) != 0;
     if (precision == EMMA_PREC_BF16X4 && p.rev && !weighted) {
-        // one 8-wave workgroup per CU: 256 slices make every launch of
-        // k_rev_bf16 a whole number of rounds over the 256 CUs
+        // one 8-wave workgroup per CU: 256 slices make the launch of
+        // k_rev_bf16 a whole number of rounds (one per task) over the 256 CUs
         const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
         p.n_slices = (int)(cap < 256 ? cap : 256);
     } else {
