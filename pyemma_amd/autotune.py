@@ -5,7 +5,9 @@ workgroup x workgroups per CU, emma_assign_variants).  Which one is fastest
 depends on the dimension (register footprint of the frame fragments, size of
 a center tile) and on the number of centers (tiles per frame block).  The
 first call for a (dimension, center count) bucket times every variant on a
-sample of the caller's own frames with HIP events on the current stream,
+sample of the caller's own frames (up to 2^21 frames: smaller samples fill
+the device for only a few rounds, where the variants tie and the full-size
+winner is not visible) with HIP events on the current stream,
 selects the fastest through emma_assign_select, and remembers the choice for
 the rest of the process (a lock guards the table; the library's own table is
 mutex-guarded).  Variants differ only in speed: labels are identical.
@@ -38,7 +40,7 @@ def tuned():
         return dict(_tuned)
 
 
-def tune_assign(Xt, Ct, mode=_native.EMMA_ASSIGN_BF16X3_RECHECK, sample=131072, reps=2):
+def tune_assign(Xt, Ct, mode=_native.EMMA_ASSIGN_BF16X3_RECHECK, sample=2097152, reps=2):
     """Select the fastest assignment tile variant for these sizes (once per bucket)."""
     if mode != _native.EMMA_ASSIGN_BF16X3_RECHECK:
         return None
