@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--lag", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 GPU comparison")
+    ap.add_argument("--no-hbm-legs", action="store_true", help="skip the small-d assignment legs")
     ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
     ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving leg (0: skip)")
@@ -192,9 +193,12 @@ def main():
         cpu, parity = cpu_baseline(args, X, offsets, C, labels, l)
 
     serving = None
-    if args.serve_seconds > 0:
+    if args.serve_seconds > 0 or not args.no_hbm_legs:
         del X, sub, labels
         torch.cuda.empty_cache()
+    if not args.no_hbm_legs:
+        kern.update(assign_hbm_legs())
+    if args.serve_seconds > 0:
         serving = serving_leg(args)
 
     out = {
@@ -262,6 +266,35 @@ def adjacent_legs(args, X, off_dev, offsets, labels, xyz):
     log("featurize %.2f ms, counts %.2f ms, vamp moments %.2f ms"
         % (out["featurize"]["ms"], out["counts"]["ms"], out["vamp_moments"]["ms"]))
     torch.cuda.synchronize()
+    return out
+
+
+def assign_hbm_legs(n=10_000_000):
+    """Assignment where it is HBM-bound by its algorithmic work (small d and k: the
+    serving models' d = 64, k = 256 and the batched workload's d = 128, k = 512),
+    10^7 Gaussian-mixture frames each; bytes = (4d + 4) N + 4 k d per call."""
+    import torch
+
+    from pyemma_amd import _native, autotune
+    from pyemma_amd.synthetic import gaussian_mixture
+
+    out = {}
+    for d, k in ((64, 256), (128, 512)):
+        X = gaussian_mixture(n, d, seed=60 + d, device=_native.device())
+        rng = np.random.default_rng(d)
+        C = X[torch.as_tensor(np.sort(rng.choice(n, size=k, replace=False)), device=X.device)].contiguous()
+        variant = autotune.tune_assign(X, C)
+        lab = torch.empty(n, dtype=torch.int32, device=X.device)
+        nre = torch.zeros(1, dtype=torch.int64, device=X.device)
+        ms = _timed(lambda: _native.assign(X, C, labels=lab, n_rechecked=nre))
+        nbytes = (4 * d + 4) * n + 4 * k * d
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out["assign_d%d_k%d" % (d, k)] = {"ms": ms, "bytes": nbytes, "gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+                                          "tile_variant": variant, "rechecked_frames": int(nre.item()),
+                                          "algorithmic_tflops": 2.0 * d * k * n / (ms * 1e-3) / 1e12}
+        log("assign d=%d k=%d: %.2f ms, %.0f GB/s (%.0f %% of HBM)" % (d, k, ms, gbs, 100 * gbs / PEAK_HBM_GBS))
+        del X, C, lab
+        torch.cuda.empty_cache()
     return out
 
 

@@ -274,7 +274,59 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
     }
 }
 
-template <int KSTEPS, int NW, int MINB>
+// Label of one frame from the running state of its two half-waves (lane and
+// lane ^ 32 hold the same frame, different centers), and whether it needs
+// the exact recheck: the frame is decided when (1 - g) times the smallest
+// lower bound over the other candidates exceeds (1 + g) times the upper bound
+// of the best (FP64 arithmetic).  Lanes < 32 write.
+__device__ __forceinline__ void decide_frame(int lane, int64_t fr, bool fvalid, float tm, float lt, float l1,
+                                             float l2, int imin, int k, float na, float sa, float CE, float gref,
+                                             const float *__restrict__ bn, int32_t *__restrict__ labels,
+                                             int64_t *__restrict__ list, int64_t *__restrict__ list_count) {
+    {
+        const float ot = __shfl_xor(tm, 32, 64);
+        const int oi = __shfl_xor(imin, 32, 64);
+        const float olt = __shfl_xor(lt, 32, 64);
+        const float o1 = __shfl_xor(l1, 32, 64);
+        const float o2 = __shfl_xor(l2, 32, 64);
+        if (ot < tm || (ot == tm && oi < imin)) {
+            tm = ot;
+            imin = oi;
+            lt = olt;
+        }
+        l2 = fminf(fminf(l2, o2), fmaxf(l1, o1));
+        l1 = fminf(l1, o1);
+    }
+    if (lane < 32 && fvalid && imin >= k) {
+        // non-finite distances: leave it to the exact loop
+        labels[fr] = 0;
+        const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
+        list[pos] = fr;
+    } else if (lane < 32 && fvalid) {
+        // smallest lower bound over the other candidates: if the winner's own
+        // bound is the smallest, the second smallest (a tie of two candidates
+        // at lb1 makes lb2 == lb1), otherwise lb1
+        const double lbo = (lt == l1) ? (double)l2 : (double)l1;
+        const double bnb = (double)bn[imin];
+        const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
+        const double upper = ((double)na + (double)tm + E1) * (1.0 + (double)gref);
+        const double lower = ((double)na - (double)CE * (double)sa * (double)sa + lbo) * (1.0 - (double)gref);
+        labels[fr] = imin;
+        if (!(lower > upper)) {
+            const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
+            list[pos] = fr;
+        }
+    }
+}
+
+// center index of the element code of a 32x32 output tile: tile t, element e
+// (lane l holds centers 8 (e >> 2) + 4 (l >> 5) + (e & 3))
+__device__ __forceinline__ int center_of_code(int code, int l) {
+    const int tt = code >> 4, e = code & 15;
+    return tt * 32 + 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+}
+
+template <int KSTEPS, int NW, int MINB, int FB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float CE,
                                                             float gref, int32_t *__restrict__ labels,
@@ -304,9 +356,16 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t frame = ((int64_t)blockIdx.x * NW + wave) * 32 + (lane & 31);
-    const bool fvalid = frame < n;
-    const int64_t frow = (fvalid ? frame : 0) * (int64_t)d;  // always a valid row
+    // FB blocks of 32 frames per wave: every center fragment read from LDS
+    // feeds FB MFMAs, and a tile's copies are hidden behind FB times the work
+    int64_t frame[FB], frow[FB];
+    bool fvalid[FB];
+#pragma unroll
+    for (int b = 0; b < FB; ++b) {
+        frame[b] = (((int64_t)blockIdx.x * NW + wave) * FB + b) * 32 + (lane & 31);
+        fvalid[b] = frame[b] < n;
+        frow[b] = (fvalid[b] ? frame[b] : 0) * (int64_t)d;  // always a valid row
+    }
     const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
     const int ntiles = k_pad / 32;
 
@@ -328,54 +387,96 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     fetch(0, -1, s_img0);
 
     // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo (branch-free) ----
-    bf16x8 bhi[KSTEPS], blo[KSTEPS];
-    float na_part = 0.0f;
+    bf16x8 bhi[FB][KSTEPS], blo[FB][KSTEPS];
+    float na[FB], sa[FB];
+    // a lane's 8 dims of a K-step are consecutive in the frame row: with d a
+    // multiple of 4 they are two 16-byte loads, each wholly inside the row or
+    // wholly in the padding (loaded from the row's last 4 dims and masked).
+    // Only for small K-step counts: with many steps the compiler keeps every
+    // loaded vector live at once and runs out of registers.
+    const bool vec = KSTEPS <= 8 && (d & 3) == 0;
 #pragma unroll
-    for (int s = 0; s < KSTEPS; ++s) {
-        uint32_t h[8], l[8];
+    for (int b = 0; b < FB; ++b) {
+        float na_part = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int i = s * 16 + kh + q;
-            const bool ok = fvalid && i < d;
-            // both loads unconditional (clamped row index; mu holds 16 KSTEPS
-            // entries) and the padding zeroed by a bit mask, so that no branch
-            // serializes the loads
-            const float xv = X[frow + (i < d ? i : d - 1)];
-            const float mv = cp.mu[i];
-            const uint32_t keep = ok ? 0xffffffffu : 0u;
-            const float a = __uint_as_float(__float_as_uint(__fsub_rn(xv, mv)) & keep);
-            na_part = __fadd_rn(na_part, __fmul_rn(a, a));
-            split_bf16(a, h[q], l[q]);
+        for (int s = 0; s < KSTEPS; ++s) {
+            uint32_t h[8], l[8];
+            float xs[8], ms[8];
+            if (vec) {
+#pragma unroll
+                for (int hv = 0; hv < 2; ++hv) {
+                    const int i0 = s * 16 + kh + 4 * hv;
+                    const float4 x4 = *reinterpret_cast<const float4 *>(X + frow[b] + (i0 < d ? i0 : d - 4));
+                    const float4 m4 = *reinterpret_cast<const float4 *>(cp.mu + i0);
+                    xs[4 * hv] = x4.x;
+                    xs[4 * hv + 1] = x4.y;
+                    xs[4 * hv + 2] = x4.z;
+                    xs[4 * hv + 3] = x4.w;
+                    ms[4 * hv] = m4.x;
+                    ms[4 * hv + 1] = m4.y;
+                    ms[4 * hv + 2] = m4.z;
+                    ms[4 * hv + 3] = m4.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int i = s * 16 + kh + q;
+                    xs[q] = X[frow[b] + (i < d ? i : d - 1)];
+                    ms[q] = cp.mu[i];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int i = s * 16 + kh + q;
+                const bool ok = fvalid[b] && i < d;
+                const uint32_t keep = ok ? 0xffffffffu : 0u;
+                const float a = __uint_as_float(__float_as_uint(__fsub_rn(xs[q], ms[q])) & keep);
+                na_part = __fadd_rn(na_part, __fmul_rn(a, a));
+                split_bf16(a, h[q], l[q]);
+            }
+            uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
+            uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
+            bhi[b][s] = __builtin_bit_cast(bf16x8, hv);
+            blo[b][s] = __builtin_bit_cast(bf16x8, lv);
         }
-        uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
-        uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
-        bhi[s] = __builtin_bit_cast(bf16x8, hv);
-        blo[s] = __builtin_bit_cast(bf16x8, lv);
+        na[b] = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
+        sa[b] = sqrtf(na[b]) * (1.0f + 1e-6f);  // |a| rounded up
     }
-    const float na = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
-    const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
 
-    // running best d~ (tmin at element code icode = 16 t + e), the lower bound
-    // of that candidate (lbt) and the two smallest lower bounds over all
-    // candidates (lb1 <= lb2, as a multiset)
-    float tmin = INFINITY, lbt = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
-    int icode = 0x7fffffff;
+    // per frame block: running best d~ (tmin at element code icode = 16 t + e),
+    // the lower bound of that candidate (lbt) and the two smallest lower bounds
+    // over all candidates (lb1 <= lb2, as a multiset)
+    float tmin[FB], lbt[FB], lb1[FB], lb2[FB];
+    int icode[FB];
+#pragma unroll
+    for (int b = 0; b < FB; ++b) {
+        tmin[b] = lbt[b] = lb1[b] = lb2[b] = INFINITY;
+        icode[b] = 0x7fffffff;
+    }
 
-    auto chain = [&](const uint8_t *img) __attribute__((always_inline)) -> floatx16 {
+    struct AccFB {
+        floatx16 v[FB];
+    };
+    auto chain = [&](const uint8_t *img) __attribute__((always_inline)) -> AccFB {
         const uint16_t *ah_base = reinterpret_cast<const uint16_t *>(img) + (lane & 31) * PITCH + kh;
         const uint16_t *al_base = ah_base + 32 * PITCH;
-        floatx16 acc = {};
+        AccFB acc;
+#pragma unroll
+        for (int b = 0; b < FB; ++b) acc.v[b] = floatx16{};
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
             const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
             const bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base + s * 16);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo[s], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[s], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[s], acc, 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo[b][s], acc.v[b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[b][s], acc.v[b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[b][s], acc.v[b], 0, 0, 0);
         }
         return acc;
     };
-    auto epilogue = [&](const floatx16 &acc, const uint8_t *img, int t) __attribute__((always_inline)) {
+    auto epilogue = [&](const AccFB &acc, const uint8_t *img, int t) __attribute__((always_inline)) {
         const float *cst = reinterpret_cast<const float *>(img + MAT);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -387,26 +488,28 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
             const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const float p = acc[4 * g + h];
-                const float tv = fmaf(-2.0f, p, nba[h]);
-                const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
-                const bool c1 = tv < tmin;  // strict: the first minimum wins
-                tmin = c1 ? tv : tmin;
-                icode = c1 ? t * 16 + 4 * g + h : icode;
-                lbt = c1 ? lb : lbt;
-                lb2 = fminf(lb2, fmaxf(lb, lb1));
-                lb1 = fminf(lb1, lb);
-            }
+            for (int b = 0; b < FB; ++b)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float p = acc.v[b][4 * g + h];
+                    const float tv = fmaf(-2.0f, p, nba[h]);
+                    const float lb = fmaf(-sa[b], ra[h], fmaf(-2.0f, p, qa[h]));
+                    const bool c1 = tv < tmin[b];  // strict: the first minimum wins
+                    tmin[b] = c1 ? tv : tmin[b];
+                    icode[b] = c1 ? t * 16 + 4 * g + h : icode[b];
+                    lbt[b] = c1 ? lb : lbt[b];
+                    lb2[b] = fminf(lb2[b], fmaxf(lb, lb1[b]));
+                    lb1[b] = fminf(lb1[b], lb);
+                }
         }
     };
     // MFMAs of one tile interleaved with the epilogue of the previous one
     auto interleave = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operand reads
-            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);  // 3 MFMAs
-            __builtin_amdgcn_sched_group_barrier(0x002, 9, 0);  // epilogue VALU
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);       // operand reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 3 * FB, 0);  // 3 FB MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x002, 9 * FB, 0);  // epilogue VALU
         }
     };
 
@@ -428,12 +531,12 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         wait_vmcnt(copies(1));
         __syncthreads();
         fetch(2 < ntiles ? 2 : -1, 1 < ntiles ? 1 : -1, s_img2);
-        floatx16 acc = chain(s_img0);
+        AccFB acc = chain(s_img0);
         auto iter = [&](int t, uint8_t *cur, uint8_t *tgt) __attribute__((always_inline)) {
             wait_vmcnt(copies(t + 1));
             __syncthreads();
             fetch(t + 2 < ntiles ? t + 2 : -1, t + 1 < ntiles ? t + 1 : -1, tgt);
-            const floatx16 nacc = chain(cur);
+            const AccFB nacc = chain(cur);
             epilogue(acc, cur, t - 1);
             interleave();
             acc = nacc;
@@ -457,14 +560,14 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     fetch(ntiles > 1 ? 1 : -1, 0, s_img1);
-    floatx16 acc = chain(s_img0);
+    AccFB acc = chain(s_img0);
     int t = 1;
     while (t < ntiles) {
         __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
         fetch(t + 1 < ntiles ? t + 1 : -1, t, s_img0);
         {
-            const floatx16 nacc = chain(s_img1);
+            const AccFB nacc = chain(s_img1);
             epilogue(acc, s_img1, t - 1);
             interleave();
             acc = nacc;
@@ -474,7 +577,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         __syncthreads();
         fetch(t + 1 < ntiles ? t + 1 : -1, t, s_img1);
         {
-            const floatx16 nacc = chain(s_img0);
+            const AccFB nacc = chain(s_img0);
             epilogue(acc, s_img0, t - 1);
             interleave();
             acc = nacc;
@@ -486,48 +589,190 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     epilogue(acc, (ntiles & 1) ? s_img1 : s_img0, ntiles - 1);
     }
 
-    // center index of the element code: tile t, element e of the 32x32 output
-    // (lane l holds centers 8 (e >> 2) + 4 (l >> 5) + (e & 3))
-    auto center_of = [&](int code, int l) -> int {
-        const int tt = code >> 4, e = code & 15;
-        return tt * 32 + 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
-    };
-    int imin = icode == 0x7fffffff ? 0x7fffffff : center_of(icode, lane);
-
-    // ---- merge the two half-waves that hold the same frame ----
-    {
-        const float ot = __shfl_xor(tmin, 32, 64);
-        const int oi = __shfl_xor(imin, 32, 64);
-        const float olt = __shfl_xor(lbt, 32, 64);
-        const float o1 = __shfl_xor(lb1, 32, 64);
-        const float o2 = __shfl_xor(lb2, 32, 64);
-        if (ot < tmin || (ot == tmin && oi < imin)) {
-            tmin = ot;
-            imin = oi;
-            lbt = olt;
+    // per frame block: merge the two half-waves that hold the same frame, then
+    // decide (same rule as decide_frame, written out here: the streaming
+    // kernel's register allocation is sensitive to the call form)
+#pragma unroll
+    for (int b = 0; b < FB; ++b) {
+        float tm = tmin[b], lt = lbt[b], l1 = lb1[b], l2 = lb2[b];
+        int imin = icode[b] == 0x7fffffff ? 0x7fffffff : center_of_code(icode[b], lane);
+        {
+            const float ot = __shfl_xor(tm, 32, 64);
+            const int oi = __shfl_xor(imin, 32, 64);
+            const float olt = __shfl_xor(lt, 32, 64);
+            const float o1 = __shfl_xor(l1, 32, 64);
+            const float o2 = __shfl_xor(l2, 32, 64);
+            if (ot < tm || (ot == tm && oi < imin)) {
+                tm = ot;
+                imin = oi;
+                lt = olt;
+            }
+            l2 = fminf(fminf(l2, o2), fmaxf(l1, o1));
+            l1 = fminf(l1, o1);
         }
-        lb2 = fminf(fminf(lb2, o2), fmaxf(lb1, o1));
-        lb1 = fminf(lb1, o1);
-    }
-    if (lane < 32 && fvalid && imin >= k) {
-        // non-finite distances: leave it to the exact loop
-        labels[frame] = 0;
-        const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-        list[pos] = frame;
-    } else if (lane < 32 && fvalid) {
-        // decision in FP64.  Smallest lower bound over the other candidates:
-        // if the winner's own bound is the smallest, the second smallest (a
-        // tie of two candidates at lb1 makes lb2 == lb1), otherwise lb1
-        const double lbo = (lbt == lb1) ? (double)lb2 : (double)lb1;
-        const double bnb = (double)cp.bn[imin];
-        const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
-        const double upper = ((double)na + (double)tmin + E1) * (1.0 + (double)gref);
-        const double lower = ((double)na - (double)CE * (double)sa * (double)sa + lbo) * (1.0 - (double)gref);
-        labels[frame] = imin;
-        if (!(lower > upper)) {
+        const int64_t fr = frame[b];
+        if (lane < 32 && fvalid[b] && imin >= k) {
+            labels[fr] = 0;  // non-finite distances: leave it to the exact loop
             const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-            list[pos] = frame;
+            list[pos] = fr;
+        } else if (lane < 32 && fvalid[b]) {
+            const double lbo = (lt == l1) ? (double)l2 : (double)l1;
+            const double bnb = (double)cp.bn[imin];
+            const double E1 = (double)CE * ((double)sa[b] + bnb) * ((double)sa[b] + bnb);
+            const double upper = ((double)na[b] + (double)tm + E1) * (1.0 + (double)gref);
+            const double lower =
+                ((double)na[b] - (double)CE * (double)sa[b] * (double)sa[b] + lbo) * (1.0 - (double)gref);
+            labels[fr] = imin;
+            if (!(lower > upper)) {
+                const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
+                list[pos] = fr;
+            }
         }
+    }
+}
+
+// Resident-center form for small k d (every center tile image fits in LDS):
+// one workgroup per CU copies all tile images into LDS once, then each wave
+// walks 32-frame blocks with a grid stride, running every tile from LDS with
+// no further copies or barriers.  Where the streaming kernel waits for a
+// tile's copies every few MFMAs (few K-steps per tile), this one only streams
+// the frames (the next block's raw values are loaded while the current block
+// runs its tiles), so its time is set by the frames' HBM traffic, the MFMAs
+// and the epilogue.
+constexpr int RES_LDS = 144 * 1024;  // bytes of resident tile images
+
+__host__ __device__ constexpr int res_max_tiles(int d_pad) { return RES_LDS / tile_img_bytes(d_pad); }
+
+template <int KSTEPS, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restrict__ X, int64_t n, int d,
+                                                           CenterPack cp, int k, int k_pad, float CE, float gref,
+                                                           int32_t *__restrict__ labels,
+                                                           int64_t *__restrict__ list,
+                                                           int64_t *__restrict__ list_count,
+                                                           const int32_t *__restrict__ done) {
+    if (done && *done) return;
+    constexpr int DP = KSTEPS * 16;
+    constexpr int PITCH = tile_pitch(DP);
+    constexpr int IMG = tile_img_bytes(DP);
+    constexpr int MAT = 128 * PITCH;
+    constexpr int NT = NW * 64;
+    __shared__ __align__(16) uint8_t s_res[RES_LDS];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int kh = (lane >> 5) * 8;
+    const int ntiles = k_pad / 32;  // ntiles * IMG <= RES_LDS (checked by the host)
+
+    // every tile image (hi, lo and the constants nb, q, r) in one asynchronous copy
+    const int nch = ntiles * (IMG / 16);
+    for (int c0 = wave * 64; c0 < nch; c0 += NT)
+        if (c0 + lane < nch)
+            __builtin_amdgcn_global_load_lds(cp.img + (int64_t)(c0 + lane) * 16, (lds_ptr_t)(s_res + c0 * 16), 16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+
+    auto chain = [&](const uint8_t *img, const bf16x8 *bhi, const bf16x8 *blo) __attribute__((always_inline)) -> floatx16 {
+        const uint16_t *ah_base = reinterpret_cast<const uint16_t *>(img) + (lane & 31) * PITCH + kh;
+        const uint16_t *al_base = ah_base + 32 * PITCH;
+        floatx16 acc = {};
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base + s * 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo[s], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[s], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[s], acc, 0, 0, 0);
+        }
+        return acc;
+    };
+
+    const int64_t nblk = (n + 31) / 32;
+    const int64_t stride = (int64_t)gridDim.x * NW;
+    // raw frame values of the wave's next block are loaded while the current
+    // block runs its tiles (lanes: frame lane & 31, dims s * 16 + kh .. + 7)
+    const bool vec = KSTEPS <= 8 && (d & 3) == 0;
+    float xr[KSTEPS][8];
+    auto load_raw = [&](int64_t b) __attribute__((always_inline)) {
+        const int64_t fr = b * 32 + (lane & 31);
+        const int64_t row = (fr < n ? fr : 0) * (int64_t)d;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            if (vec) {
+#pragma unroll
+                for (int hv = 0; hv < 2; ++hv) {
+                    const int i0 = s * 16 + kh + 4 * hv;
+                    const float4 x4 = *reinterpret_cast<const float4 *>(X + row + (i0 < d ? i0 : d - 4));
+                    xr[s][4 * hv] = x4.x;
+                    xr[s][4 * hv + 1] = x4.y;
+                    xr[s][4 * hv + 2] = x4.z;
+                    xr[s][4 * hv + 3] = x4.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int i = s * 16 + kh + q;
+                    xr[s][q] = X[row + (i < d ? i : d - 1)];
+                }
+            }
+        }
+    };
+    int64_t blk = (int64_t)blockIdx.x * NW + wave;
+    if (blk < nblk) load_raw(blk);
+    for (; blk < nblk; blk += stride) {
+        const int64_t frame = blk * 32 + (lane & 31);
+        const bool fvalid = frame < n;
+        bf16x8 bhi[KSTEPS], blo[KSTEPS];
+        float na_part = 0.0f;
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            uint32_t h[8], l[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int i = s * 16 + kh + q;
+                const uint32_t keep = (fvalid && i < d) ? 0xffffffffu : 0u;
+                const float a = __uint_as_float(__float_as_uint(__fsub_rn(xr[s][q], cp.mu[i])) & keep);
+                na_part = __fadd_rn(na_part, __fmul_rn(a, a));
+                split_bf16(a, h[q], l[q]);
+            }
+            bhi[s] = __builtin_bit_cast(bf16x8, make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7])));
+            blo[s] = __builtin_bit_cast(bf16x8, make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7])));
+        }
+        const float na = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
+        const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
+        if (blk + stride < nblk) load_raw(blk + stride);
+
+        float tmin = INFINITY, lbt = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
+        int icode = 0x7fffffff;
+        for (int t = 0; t < ntiles; ++t) {
+            const uint8_t *img = s_res + t * IMG;
+            const floatx16 acc = chain(img, bhi, blo);
+            const float *cst = reinterpret_cast<const float *>(img + MAT);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int c0 = 8 * g + 4 * (lane >> 5);
+                const float4 nbv = *reinterpret_cast<const float4 *>(&cst[c0]);
+                const float4 qv = *reinterpret_cast<const float4 *>(&cst[32 + c0]);
+                const float4 rv = *reinterpret_cast<const float4 *>(&cst[64 + c0]);
+                const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
+                const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+                const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float p = acc[4 * g + h];
+                    const float tv = fmaf(-2.0f, p, nba[h]);
+                    const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
+                    const bool c1 = tv < tmin;  // strict: the first minimum wins
+                    tmin = c1 ? tv : tmin;
+                    icode = c1 ? t * 16 + 4 * g + h : icode;
+                    lbt = c1 ? lb : lbt;
+                    lb2 = fminf(lb2, fmaxf(lb, lb1));
+                    lb1 = fminf(lb1, lb);
+                }
+            }
+        }
+        const int imin = icode == 0x7fffffff ? 0x7fffffff : center_of_code(icode, lane);
+        decide_frame(lane, frame, fvalid, tmin, lbt, lb1, lb2, imin, k, na, sa, CE, gref, cp.bn, labels, list,
+                     list_count);
     }
 }
 
@@ -585,20 +830,28 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     return w;
 }
 
-// Tile variants (waves per workgroup, workgroups per CU) of the MFMA kernel
-// for each K-step count: every variant fits the registers and the two LDS
-// tile buffers of its occupancy.  Variant 0 is the default; the autotuner
-// (emma_assign_select) may pick another per (K steps, k) bucket.
+// Tile variants of the MFMA kernel for each K-step count: every variant fits
+// the registers and the LDS tile buffers of its occupancy.  Several 32-frame
+// blocks per wave (small d, where the fragments of a block take few
+// registers) reuse each center fragment for more MFMAs and hide a tile's
+// copies behind more work.  Variant 0 is the default; for K steps <= 8 and
+// k small enough that every tile image fits in LDS it runs the
+// resident-center kernel instead.  The autotuner (emma_assign_select) may
+// pick another per (K steps, k) bucket.
+// (waves per workgroup, workgroups per CU, 32-frame blocks per wave)
 struct Variant {
-    int nw, minb;
+    int nw, minb, fb;
 };
-static const Variant kVarSmall[] = {{4, 2}, {8, 1}, {16, 1}};  // KSTEPS <= 4
-static const Variant kVarMid[] = {{4, 2}, {8, 1}};             // 8 .. 19
-static const Variant kVarLarge[] = {{4, 1}};                   // 24, 32
+static const Variant kVarSmall[] = {{4, 2, 1}, {8, 1, 1}, {16, 1, 1}, {4, 2, 2}, {8, 1, 2}, {4, 1, 4}};  // KSTEPS <= 4
+static const Variant kVarK8[] = {{4, 2, 1}, {8, 1, 1}, {4, 1, 2}};                                      // KSTEPS = 8
+static const Variant kVarMid[] = {{4, 2, 1}, {8, 1, 1}};                                              // 12 .. 19
+static const Variant kVarLarge[] = {{4, 1, 1}};                                                       // 24, 32
 
 static int n_variants(int ks) {
-    return ks <= 4 ? (int)(sizeof(kVarSmall) / sizeof(Variant))
-                   : (ks <= 19 ? (int)(sizeof(kVarMid) / sizeof(Variant)) : (int)(sizeof(kVarLarge) / sizeof(Variant)));
+    if (ks <= 4) return (int)(sizeof(kVarSmall) / sizeof(Variant));
+    if (ks <= 8) return (int)(sizeof(kVarK8) / sizeof(Variant));
+    if (ks <= 19) return (int)(sizeof(kVarMid) / sizeof(Variant));
+    return (int)(sizeof(kVarLarge) / sizeof(Variant));
 }
 
 // process-wide selection table: [K-step class][ceil(log2(k_pad / 32))]
@@ -624,22 +877,51 @@ static int selected_variant(int ks, int k_pad) {
     return v < n_variants(ks) ? v : 0;
 }
 
-template <int KS, int NW, int MINB>
+template <int KS, int NW, int MINB, int FB = 1>
 static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE, float gref,
                      int32_t *labels, const int32_t *done, hipStream_t s) {
-    const int64_t blocks = cdiv(n, 32 * NW);
-    k_assign_mfma<KS, NW, MINB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
-                                                                     w.list, w.count, done);
+    const int64_t blocks = cdiv(n, 32 * NW * FB);
+    k_assign_mfma<KS, NW, MINB, FB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
+                                                                         w.list, w.count, done);
+}
+
+static int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus;
 }
 
 template <int KS>
 static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     const int v = selected_variant(KS, k_pad);
+    if constexpr (KS <= 8) {
+        // variant 0 with every tile image resident in LDS when they fit
+        if (v == 0 && k_pad / 32 <= res_max_tiles(KS * 16)) {
+            constexpr int NW = KS <= 4 ? 8 : 4;  // 8 K-steps: 4 waves, the registers of one wave per SIMD
+            int64_t blocks = cdiv(n, 32 * NW);
+            const int64_t cap = device_cus();
+            if (blocks > cap) blocks = cap;
+            k_assign_res<KS, NW><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
+                                                                    w.list, w.count, done);
+            return;
+        }
+    }
     if constexpr (KS <= 4) {
         switch (v) {
             case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
             case 2: launch_v<KS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 3: launch_v<KS, 4, 2, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 4: launch_v<KS, 8, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 5: launch_v<KS, 4, 1, 4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+        }
+    } else if constexpr (KS <= 8) {
+        switch (v) {
+            case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 2: launch_v<KS, 4, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
             default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
         }
     } else if constexpr (KS <= 19) {

@@ -87,3 +87,20 @@ def test_deterministic_repeat():
     b, nb = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
     np.testing.assert_array_equal(a, b)
     assert na == nb
+
+
+@pytest.mark.parametrize("d,k", [(20, 70), (64, 256), (128, 200), (128, 300), (200, 100), (300, 1000), (512, 64)])
+def test_every_tile_variant_bit_exact(d, k):
+    """Each tile variant (waves per workgroup, workgroups per CU, frame blocks per wave) gives the
+    oracle's labels, including a ragged last workgroup."""
+    L = _native.lib()
+    n = 4 * 1024 + 77
+    X, C = _data(n, d, k, seed=d * 7 + k)
+    ref = on.assign(X, C, nthreads=4)
+    try:
+        for v in range(L.emma_assign_variants(d)):
+            _native.check(L.emma_assign_select(d, k, v), "emma_assign_select")
+            lab, _ = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+            np.testing.assert_array_equal(lab, ref, err_msg="variant %d" % v)
+    finally:
+        _native.check(L.emma_assign_select(d, k, 0), "emma_assign_select")
