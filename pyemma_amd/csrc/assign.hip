@@ -165,6 +165,7 @@ struct CenterPack {
     uint8_t *img;  // k_pad / 32 tile images of img_bytes each
     float *bn;     // [k_pad] |b| rounded up
     float *mu;     // [d_pad] shift
+    float *bmax;   // max_j |b_j| (zeroed before the packing, then an atomic max)
 };
 
 __host__ __device__ constexpr int tile_pitch(int d_pad) { return d_pad + 8; }
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C
             r = 0.0f;
         }
         cp.bn[j] = bn;
+        if (j < k) atomicMax(reinterpret_cast<unsigned int *>(cp.bmax), __float_as_uint(bn));  // bn >= 0
         cst[j & 31] = nb;
         cst[32 + (j & 31)] = q;
         cst[64 + (j & 31)] = r;
@@ -276,41 +278,39 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 
 // Label of one frame from the running state of its two half-waves (lane and
 // lane ^ 32 hold the same frame, different centers), and whether it needs
-// the exact recheck: the frame is decided when (1 - g) times the smallest
-// lower bound over the other candidates exceeds (1 + g) times the upper bound
-// of the best (FP64 arithmetic).  Lanes < 32 write.
-__device__ __forceinline__ void decide_frame(int lane, int64_t fr, bool fvalid, float tm, float lt, float l1,
-                                             float l2, int imin, int k, float na, float sa, float CE, float gref,
-                                             const float *__restrict__ bn, int32_t *__restrict__ labels,
-                                             int64_t *__restrict__ list, int64_t *__restrict__ list_count) {
+// the exact recheck (lanes < 32 write).  The rule of the streaming kernel,
+// but from the two smallest d~ (tmin of the winner, t2 of the runner-up) and
+// one bound for every other candidate, E_max = CE (|a| + max_j
+// |b_j|)^2 >= E_j: for j != best, d_j >= d~_j - E_j >= t2 - E_max, and d_best
+// <= tmin + E_best.  Looser than the per-candidate form (about a tenth more
+// frames go to the recheck) but the epilogue tracks 2 values per element
+// instead of 4: the resident kernel, whose time is set by its vector ALU
+// work, uses it (measured faster there; in the streaming kernel it is not).
+__device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fvalid, float tm, float t2, int imin,
+                                                   int k, float na, float sa, float CE, float gref, float bmax,
+                                                   const float *__restrict__ bn, int32_t *__restrict__ labels,
+                                                   int64_t *__restrict__ list, int64_t *__restrict__ list_count) {
     {
         const float ot = __shfl_xor(tm, 32, 64);
         const int oi = __shfl_xor(imin, 32, 64);
-        const float olt = __shfl_xor(lt, 32, 64);
-        const float o1 = __shfl_xor(l1, 32, 64);
-        const float o2 = __shfl_xor(l2, 32, 64);
+        const float o2 = __shfl_xor(t2, 32, 64);
+        // second smallest of the union of {tm, t2} and {ot, o2}
+        t2 = fminf(fminf(t2, o2), fmaxf(tm, ot));
         if (ot < tm || (ot == tm && oi < imin)) {
             tm = ot;
             imin = oi;
-            lt = olt;
         }
-        l2 = fminf(fminf(l2, o2), fmaxf(l1, o1));
-        l1 = fminf(l1, o1);
     }
     if (lane < 32 && fvalid && imin >= k) {
-        // non-finite distances: leave it to the exact loop
-        labels[fr] = 0;
+        labels[fr] = 0;  // non-finite distances: leave it to the exact loop
         const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
         list[pos] = fr;
     } else if (lane < 32 && fvalid) {
-        // smallest lower bound over the other candidates: if the winner's own
-        // bound is the smallest, the second smallest (a tie of two candidates
-        // at lb1 makes lb2 == lb1), otherwise lb1
-        const double lbo = (lt == l1) ? (double)l2 : (double)l1;
         const double bnb = (double)bn[imin];
         const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
+        const double Em = (double)CE * ((double)sa + (double)bmax) * ((double)sa + (double)bmax);
         const double upper = ((double)na + (double)tm + E1) * (1.0 + (double)gref);
-        const double lower = ((double)na - (double)CE * (double)sa * (double)sa + lbo) * (1.0 - (double)gref);
+        const double lower = ((double)na + (double)t2 - Em) * (1.0 - (double)gref);
         labels[fr] = imin;
         if (!(lower > upper)) {
             const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
@@ -670,6 +670,7 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
             __builtin_amdgcn_global_load_lds(cp.img + (int64_t)(c0 + lane) * 16, (lds_ptr_t)(s_res + c0 * 16), 16, 0, 0);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
+    const float bmax = *cp.bmax;
 
     auto chain = [&](const uint8_t *img, const bf16x8 *bhi, const bf16x8 *blo) __attribute__((always_inline)) -> floatx16 {
         const uint16_t *ah_base = reinterpret_cast<const uint16_t *>(img) + (lane & 31) * PITCH + kh;
@@ -741,7 +742,8 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
         const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
         if (blk + stride < nblk) load_raw(blk + stride);
 
-        float tmin = INFINITY, lbt = INFINITY, lb1 = INFINITY, lb2 = INFINITY;
+        // the two smallest d~ (decide_frame_loose)
+        float tmin = INFINITY, t2 = INFINITY;
         int icode = 0x7fffffff;
         for (int t = 0; t < ntiles; ++t) {
             const uint8_t *img = s_res + t * IMG;
@@ -751,28 +753,20 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
             for (int g = 0; g < 4; ++g) {
                 const int c0 = 8 * g + 4 * (lane >> 5);
                 const float4 nbv = *reinterpret_cast<const float4 *>(&cst[c0]);
-                const float4 qv = *reinterpret_cast<const float4 *>(&cst[32 + c0]);
-                const float4 rv = *reinterpret_cast<const float4 *>(&cst[64 + c0]);
                 const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
-                const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
-                const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
-                    const float p = acc[4 * g + h];
-                    const float tv = fmaf(-2.0f, p, nba[h]);
-                    const float lb = fmaf(-sa, ra[h], fmaf(-2.0f, p, qa[h]));
+                    const float tv = fmaf(-2.0f, acc[4 * g + h], nba[h]);
                     const bool c1 = tv < tmin;  // strict: the first minimum wins
+                    t2 = fminf(t2, fmaxf(tv, tmin));
                     tmin = c1 ? tv : tmin;
                     icode = c1 ? t * 16 + 4 * g + h : icode;
-                    lbt = c1 ? lb : lbt;
-                    lb2 = fminf(lb2, fmaxf(lb, lb1));
-                    lb1 = fminf(lb1, lb);
                 }
             }
         }
         const int imin = icode == 0x7fffffff ? 0x7fffffff : center_of_code(icode, lane);
-        decide_frame(lane, frame, fvalid, tmin, lbt, lb1, lb2, imin, k, na, sa, CE, gref, cp.bn, labels, list,
-                     list_count);
+        decide_frame_loose(lane, frame, fvalid, tmin, t2, imin, k, na, sa, CE, gref, bmax, cp.bn, labels, list,
+                           list_count);
     }
 }
 
@@ -816,8 +810,9 @@ struct AssignWs {
 static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     AssignWs w{};
     Carver cv(ws);
-    w.count = cv.take<int64_t>(1);
+    w.count = cv.take<int64_t>(2);  // list length, then the bits of max |b_j| (both zeroed per call)
     w.list = cv.take<int64_t>(n > 0 ? (size_t)n : 1);
+    w.cp.bmax = w.count ? reinterpret_cast<float *>(w.count + 1) : nullptr;
     if (mode == EMMA_ASSIGN_BF16X3_RECHECK) {
         const int ks = pick_ksteps(d);
         const int d_pad = (ks > 0 ? ks : 1) * 16;
@@ -955,7 +950,7 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
     const int k_pad = (int)cdiv(k, 32) * 32;
     const float CE = (float)assign_error_coeff(d, d_pad);
     const float gref = (float)gamma_ref(d);
-    k_zero_i64n<<<1, 64, 0, s>>>(w.count, 1);
+    k_zero_i64n<<<1, 64, 0, s>>>(w.count, 2);
     k_center_mean<<<(unsigned)cdiv(d_pad, 64), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
     k_center_pack<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, CE, w.cp);
     EMMA_TRY(check_launch("assign center pack"));
