@@ -463,16 +463,24 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         AccFB acc;
 #pragma unroll
         for (int b = 0; b < FB; ++b) acc.v[b] = floatx16{};
+        // center operands read one K-step ahead of their MFMAs
+        bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base);
+        bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base);
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
-            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base + s * 16);
-            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base + s * 16);
+            bf16x8 ahn = ah, aln = al;
+            if (s + 1 < KSTEPS) {
+                ahn = *reinterpret_cast<const bf16x8 *>(ah_base + (s + 1) * 16);
+                aln = *reinterpret_cast<const bf16x8 *>(al_base + (s + 1) * 16);
+            }
 #pragma unroll
             for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo[b][s], acc.v[b], 0, 0, 0);
 #pragma unroll
             for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bhi[b][s], acc.v[b], 0, 0, 0);
 #pragma unroll
             for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bhi[b][s], acc.v[b], 0, 0, 0);
+            ah = ahn;
+            al = aln;
         }
         return acc;
     };
@@ -505,11 +513,12 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     };
     // MFMAs of one tile interleaved with the epilogue of the previous one
     auto interleave = [&]() __attribute__((always_inline)) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operand reads of K-step 0
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);       // operand reads
-            __builtin_amdgcn_sched_group_barrier(0x008, 3 * FB, 0);  // 3 FB MFMAs
-            __builtin_amdgcn_sched_group_barrier(0x002, 9 * FB, 0);  // epilogue VALU
+            if (s + 1 < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operands of K-step s + 1
+            __builtin_amdgcn_sched_group_barrier(0x008, 3 * FB, 0);                 // 3 FB MFMAs of K-step s
+            __builtin_amdgcn_sched_group_barrier(0x002, 9 * FB, 0);                 // epilogue VALU
         }
     };
 
