@@ -747,19 +747,20 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
             for (int c = 0; c < 2; ++c) acc[r][c] = floatx16{};
     };
     auto mfma = [&](int buf) __attribute__((always_inline)) {
+        // operand reads in the order the MFMAs consume them (lo parts first),
+        // so that the first products wait for two reads, not eight
         bf16x8 ah[2], al[2], bh[2], bl[2];
+        const int offr0 = (wr * 64 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
+        const int offc0 = (wc * 64 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            const int off = (wr * 64 + r * 32 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
-            ah[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][0][off]);
-            al[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][1][off]);
+            al[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][1][offr0 + r * 32 * RPITCH]);
+            bl[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][1][offc0 + r * 32 * RPITCH]);
         }
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int off = (wc * 64 + c * 32 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
-            bh[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][0][off]);
-            bl[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][1][off]);
-        }
+        for (int r = 0; r < 2; ++r) ah[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][0][offr0 + r * 32 * RPITCH]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) bh[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][0][offc0 + c * 32 * RPITCH]);
         // small products first
 #pragma unroll
         for (int r = 0; r < 2; ++r)
