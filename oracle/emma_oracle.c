@@ -9,7 +9,7 @@
  * The reference delegates the k-means arithmetic to deeptime
  * (pyemma/coordinates/clustering/kmeans.py:39, interface.py:164-165), whose
  * source is not part of the reference tree.  The rules restated here are the
- * ones fixed for this build and documented in the functions below:
+ * ones fixed here and documented in the functions below:
  *
  *  - assign: label(x) = argmin_j sum_i (x_i - c_ji)^2, the sum evaluated in
  *    FP32 sequentially over i with no fused multiply-add, strict '<' so the

@@ -16,7 +16,7 @@ Pinned by (tests/test_oracle_kmeans.py):
     the documented center update.
 The D^2 sampling order and RNG, and the Lloyd accumulation order, have no
 counterpart in the reference tree (deeptime is not vendored): those rules
-are this build's documented definition, parity unpinned against deeptime.
+are the documented definition used here, parity unpinned against deeptime.
 """
 import ctypes
 import os
