@@ -162,17 +162,17 @@ def main():
     log("step %.2f ms (moments %.2f ms, assign %.2f ms, host eig %.2f ms, rechecked %d)"
         % (step_s * 1e3, mom_ms, asg_ms, eig_ms, n_re))
 
-    # exact mode: FP64 MFMA moments (one call, same stream)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True, c0t=True,
-                           precision=_native.EMMA_PREC_FP64)
-    e1.record()
-    torch.cuda.synchronize()
-    fp64_ms = e0.elapsed_time(e1)
-    kern["moments_fp64"] = {"ms": fp64_ms, "algorithmic_tflop": f_mom / 1e12,
-                            "tflops": f_mom / (fp64_ms * 1e-3) / 1e12}
-    log("fp64 moments %.2f ms" % fp64_ms)
+    # the accurate modes, one call each on the same stream: FP64 tiles, and the
+    # bf16 split with two-level FP32 accumulation
+    for key, prec in (("moments_fp64", _native.EMMA_PREC_FP64), ("moments_bf16x4_acc", _native.EMMA_PREC_BF16X4_ACC)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True, c0t=True, precision=prec)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        kern[key] = {"ms": ms, "algorithmic_tflop": f_mom / 1e12, "tflops": f_mom / (ms * 1e-3) / 1e12}
+        log("%s %.2f ms" % (key, ms))
 
     kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
     del xyz
@@ -408,7 +408,8 @@ def cpu_baseline(args, X, offsets, C, labels_gpu, l_gpu):
     ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
     parity = {"labels_identical": bool(np.array_equal(lab_gpu, lab_ref)), "label_frames": na, "eig_frames": n_m}
     big = np.abs(ev_ref) >= 0.1
-    for name, prec in (("bf16x4", _native.EMMA_PREC_BF16X4), ("fp64", _native.EMMA_PREC_FP64)):
+    for name, prec in (("bf16x4", _native.EMMA_PREC_BF16X4), ("bf16x4_acc", _native.EMMA_PREC_BF16X4_ACC),
+                       ("fp64", _native.EMMA_PREC_FP64)):
         w, sx, sy, Mxx, Mxy, _ = _native.moments_lagged(Xs, off_s, args.lag, reversible=True, remove_mean=True,
                                                         c0t=True, precision=prec)
         wh = w.item()

@@ -22,6 +22,7 @@ EMMA_MOM_WEIGHTED = 16
 EMMA_PREC_FP32 = 0
 EMMA_PREC_BF16X4 = 1
 EMMA_PREC_FP64 = 2
+EMMA_PREC_BF16X4_ACC = 3
 EMMA_ASSIGN_FP32_PLAIN = 0
 EMMA_ASSIGN_BF16X3_RECHECK = 1
 EMMA_DIST_INVERSE = 1

@@ -17,7 +17,8 @@ from ._data import to_device
 
 log = logging.getLogger("pyemma_amd.coordinates.estimation.covariance.LaggedCovariance")
 
-PRECISIONS = {"bf16x4": _native.EMMA_PREC_BF16X4, "fp32": _native.EMMA_PREC_FP32, "fp64": _native.EMMA_PREC_FP64}
+PRECISIONS = {"bf16x4": _native.EMMA_PREC_BF16X4, "fp32": _native.EMMA_PREC_FP32, "fp64": _native.EMMA_PREC_FP64,
+              "bf16x4-acc": _native.EMMA_PREC_BF16X4_ACC}
 
 
 class Moments(object):

@@ -51,6 +51,7 @@ struct Params {
     int ntraj;
     int lag, stride, skip;
     int tb;               // tile edge
+    int acc2;             // reversible bf16 kernel: two-level FP32 accumulation
     int frag;             // slot layout: 1 = MFMA fragment order (TB 128), 0 = row-major
     int64_t slot_doubles; // doubles per slot: 2 matrices + 2 column-sum vectors
     int nb;               // column blocks = ceil(d / tb)
@@ -583,6 +584,7 @@ constexpr int RBT = 16;              // pairs per stage
 constexpr int RPITCH = RBT + 8;      // uint16 per column row (48 B: conflict-free 16-byte reads)
 constexpr int RFLUSH = 1024;         // pairs per FP32 accumulation span
 constexpr int RSPAN = RFLUSH / RBT;  // stages per span (even)
+constexpr int RSUB = 4;              // stages per FP32 sub-span (even, divides RSPAN)
 
 struct RevLds {
     uint16_t op[2][4][2][RTB * RPITCH];  // [buffer][U_I, U_J, V_I, V_J][hi, lo][column row]
@@ -594,7 +596,7 @@ struct PairCursor {
     int64_t p0, p1, base;
 };
 
-template <bool DIAG>
+template <bool DIAG, bool TL>
 __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&s_sum)[4][RTB], int I, int J,
                                          int task, int slice) {
     constexpr int PPT = DIAG ? 4 : 8;  // pairs per loader thread per stage
@@ -739,12 +741,32 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         }
     };
 
-    floatx16 acc[2][2];
+    // TL (two-level FP32 accumulation): the MFMAs accumulate a sub-span of
+    // RSUB stages from zero (acc), which is then added to the span sum (sacc);
+    // the span sum goes to the FP64 slot every RFLUSH pairs.  An element of a
+    // span then carries about RSUB * 4 + RSPAN / RSUB sequential FP32
+    // roundings instead of RSPAN * 4.  It costs 64 more registers per lane,
+    // beyond the budget of two waves per SIMD (the compiler spills).  Without
+    // TL, acc is the span sum itself.
+    floatx16 acc[2][2], sacc[2][2];
     auto zero_acc = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = floatx16{};
+            for (int c = 0; c < 2; ++c) {
+                acc[r][c] = floatx16{};
+                if constexpr (TL) sacc[r][c] = floatx16{};
+            }
+    };
+    auto fold = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) sacc[r][c][e] = __fadd_rn(sacc[r][c][e], acc[r][c][e]);
+                acc[r][c] = floatx16{};
+            }
     };
     auto mfma = [&](int buf) __attribute__((always_inline)) {
         // operand reads in the order the MFMAs consume them (lo parts first),
@@ -805,7 +827,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
                 asm volatile("" : "+v"(loff));  // recomputed per flush: keeps the addresses out of the loop
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
-                    const double v = (double)acc[r][c][e];
+                    const double v = TL ? (double)sacc[r][c][e] : (double)acc[r][c][e];
                     double *dst = reinterpret_cast<double *>(base + e * 512 + loff);
                     // private slot, one writer lane per element: no atomicity needed;
                     // the L2-side add keeps the old value out of registers
@@ -856,7 +878,11 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         for (int s = 0; s < n_even; s += 2) {
             half(act_tag, 0, xb, yb, 1, s + 3);  // even stage s; prepare stage s + 1
             half(act_tag, 1, xa, ya, 0, s + 4);  // odd stage s + 1; prepare stage s + 2
-            if (((s + 2) % RSPAN) == 0 || s + 2 >= n_even) {
+            const bool span_end = ((s + 2) % RSPAN) == 0 || s + 2 >= n_even;
+            if constexpr (ACT && TL) {
+                if (span_end || ((s + 2) % RSUB) == 0) fold();
+            }
+            if (span_end) {
                 if constexpr (ACT) {
                     flush(first_flush);
                     zero_acc();
@@ -888,6 +914,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
 // tasks first, then the diagonal ones), so the tasks that read the same frame
 // rows run side by side and share the XCD's L2: each frame comes from HBM
 // about once per call instead of once per kind of task.
+template <bool TL>
 __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     __shared__ __align__(16) RevLds lds;
     __shared__ double s_sum[4][RTB];
@@ -913,9 +940,9 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     for (int i = 0; i < I; ++i) task += p.nb - i;
     task += J - I;
     if (I == J)
-        rev_body<true>(p, lds, s_sum, I, J, task, slice);
+        rev_body<true, TL>(p, lds, s_sum, I, J, task, slice);
     else
-        rev_body<false>(p, lds, s_sum, I, J, task, slice);
+        rev_body<false, TL>(p, lds, s_sum, I, J, task, slice);
 }
 
 // ------------------------------------------------------------------
@@ -1069,7 +1096,10 @@ static void launch_kind_w(const Params &p, hipStream_t s) {
     const unsigned g_sym = (unsigned)(cdiv((int64_t)n_sym * p.n_slices, 8) * 8);
     if constexpr (PREC == EMMA_PREC_BF16X4 && !WT) {
         if (p.rev) {
-            k_rev_bf16<<<g_sym, RNT, 0, s>>>(p);
+            if (p.acc2)
+                k_rev_bf16<true><<<g_sym, RNT, 0, s>>>(p);
+            else
+                k_rev_bf16<false><<<g_sym, RNT, 0, s>>>(p);
             return;
         }
     }
@@ -1123,6 +1153,7 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.lag = lag;
     p.stride = stride;
     p.skip = skip;
+    p.acc2 = 0;
     p.tb = precision == EMMA_PREC_FP64 ? 64 : 128;
     p.frag = precision == EMMA_PREC_FP64 ? 0 : 1;
     p.slot_doubles = 2LL * p.tb * p.tb + 2LL * p.tb;
@@ -1163,9 +1194,32 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
 
 using namespace emma;
 
+namespace emma {
+namespace mom {
+// EMMA_PREC_BF16X4_ACC: the reversible, unweighted moments run the bf16 kernel
+// with two-level accumulation; every other kind runs the FP64 tiles, which
+// meet the same accuracy.
+static int effective_precision(int precision, uint32_t flags, int lag, int *acc2) {
+    *acc2 = 0;
+    if (precision != EMMA_PREC_BF16X4_ACC) return precision;
+    const bool rev = (flags & EMMA_MOM_SYMMETRIZE) && lag > 0;
+    if (rev && !(flags & EMMA_MOM_WEIGHTED)) {
+        *acc2 = 1;
+        return EMMA_PREC_BF16X4;
+    }
+    return EMMA_PREC_FP64;
+}
+}  // namespace mom
+}  // namespace emma
+
 extern "C" size_t emma_moments_workspace_size(int64_t n_rows, int32_t d, int32_t ntraj, uint32_t flags,
                                               int32_t precision) {
     if (d <= 0 || ntraj < 0) return 0;
+    if (precision == EMMA_PREC_BF16X4_ACC) {  // either form may run: size for both
+        const size_t a = emma_moments_workspace_size(n_rows, d, ntraj, flags, EMMA_PREC_BF16X4);
+        const size_t b = emma_moments_workspace_size(n_rows, d, ntraj, flags, EMMA_PREC_FP64);
+        return a > b ? a : b;
+    }
     mom::Plan pl, pl0;
     // lag only matters through lag == 0; size for the larger task set
     mom::make_plan(pl, nullptr, n_rows, d, nullptr, ntraj, 1, 1, 0, flags, precision, nullptr);
@@ -1180,7 +1234,8 @@ extern "C" int emma_moments_lagged_weighted(const float *X, int64_t n_rows, int3
                                             size_t workspace_bytes, void *stream) {
     EMMA_REQUIRE(d > 0 && ntraj > 0 && n_rows >= 0, "emma_moments_lagged: bad sizes d=%d ntraj=%d", d, ntraj);
     EMMA_REQUIRE(lag >= 0 && stride >= 1 && skip >= 0, "emma_moments_lagged: bad lag/stride/skip");
-    EMMA_REQUIRE(precision == EMMA_PREC_FP32 || precision == EMMA_PREC_BF16X4 || precision == EMMA_PREC_FP64,
+    EMMA_REQUIRE(precision == EMMA_PREC_FP32 || precision == EMMA_PREC_BF16X4 || precision == EMMA_PREC_FP64 ||
+                     precision == EMMA_PREC_BF16X4_ACC,
                  "emma_moments_lagged: unknown precision %d", precision);
     EMMA_REQUIRE(X && traj_offsets && w && sx && Mxx, "emma_moments_lagged: NULL pointer");
     EMMA_REQUIRE(((flags & EMMA_MOM_WEIGHTED) != 0) == (row_weights != nullptr),
@@ -1191,12 +1246,15 @@ extern "C" int emma_moments_lagged_weighted(const float *X, int64_t n_rows, int3
                  "emma_moments_lagged: lag must be positive if c0t or ctt are requested");
     EMMA_REQUIRE(!(flags & EMMA_MOM_C0T) || Mxy, "emma_moments_lagged: Mxy is NULL");
     EMMA_REQUIRE(!(flags & EMMA_MOM_CTT) || Myy, "emma_moments_lagged: Myy is NULL");
+    int acc2 = 0;
+    precision = mom::effective_precision(precision, flags, lag, &acc2);
     mom::Plan pl;
     mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, nullptr);
     EMMA_REQUIRE(workspace && workspace_bytes >= pl.ws_bytes,
                  "emma_moments_lagged: workspace too small (%zu < %zu)", workspace_bytes, pl.ws_bytes);
     mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, workspace);
     pl.p.wts = row_weights;
+    pl.p.acc2 = acc2;
     hipStream_t s = (hipStream_t)stream;
     const mom::Params &p = pl.p;
     mom::k_prep<<<1, 1024, 0, s>>>(p);

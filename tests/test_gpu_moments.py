@@ -15,7 +15,7 @@ from pyemma_amd import _native
 
 pytestmark = pytest.mark.gpu
 
-PRECS = [_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_FP32, _native.EMMA_PREC_FP64]
+PRECS = [_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_FP32, _native.EMMA_PREC_FP64, _native.EMMA_PREC_BF16X4_ACC]
 
 
 def _gpu(trajs, lag, prec, **kw):
@@ -127,6 +127,34 @@ def test_tica_eigenvalues_vs_fp64():
         else:
             assert rel[big].max() < 1e-6, (prec, rel[big].max())
             assert np.abs(ev - ev_ref).max() < 1e-5, (prec, np.abs(ev - ev_ref).max())
+
+
+def test_accurate_mode_on_distance_features():
+    """Pairwise distances of 25 Brownian particles (the 300 collinear features of the
+    200 x 50,000 workload; 2 trajectories here), lag 20: the generalized eigenproblem
+    amplifies the FP32 accumulation error of long MFMA chains.  BF16X4_ACC (two-level
+    FP32 accumulation) keeps the eigenvalues with |l| >= 0.1 within 1e-6 relative of
+    the FP64 oracle; plain BF16X4 is reported, with a loose bound."""
+    from pyemma_amd.synthetic import brownian_distances
+    t = _native.torch()
+    X, off = brownian_distances(2, 50000, seed=2, device=_native.device())
+    Xh = X.cpu().numpy()
+    lag = 20
+    trajs = [Xh[off[i]:off[i + 1]] for i in range(2)]
+    ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=True, bessel=False)
+    ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
+    big = np.abs(ev_ref) >= 0.1
+    errs = {}
+    for prec in (_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_BF16X4_ACC):
+        w, sx, _, Mxx, Mxy, _ = [None if o is None else o.cpu().numpy() for o in _native.moments_lagged(
+            X, off, lag, precision=prec, reversible=True, remove_mean=True, c0t=True)]
+        t.cuda.synchronize()
+        ev = oeig.generalized_eigenvalues(Mxx / w[0], Mxy / w[0])
+        errs[prec] = (np.abs(ev - ev_ref) / np.abs(ev_ref))[big].max()
+    print("eigenvalue rel. err (|l| >= 0.1): bf16x4 %.3g, bf16x4-acc %.3g"
+          % (errs[_native.EMMA_PREC_BF16X4], errs[_native.EMMA_PREC_BF16X4_ACC]))
+    assert errs[_native.EMMA_PREC_BF16X4_ACC] < 1e-6
+    assert errs[_native.EMMA_PREC_BF16X4] < 1e-4
 
 
 def test_deterministic_repeat():
