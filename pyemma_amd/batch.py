@@ -31,10 +31,10 @@ class BatchedDiscretizer(object):
     offsets    int64 trajectory offsets (ntraj + 1)
     lag        TICA lag time
     centers    (k, d) float32 cluster centers
-    precision  moment tile arithmetic ('bf16x4', 'fp32', 'fp64')
+    precision  moment tile arithmetic ('bf16x4-acc', 'bf16x4', 'fp32', 'fp64')
     """
 
-    def __init__(self, X, offsets, lag, centers, precision="bf16x4", reversible=True,
+    def __init__(self, X, offsets, lag, centers, precision="bf16x4-acc", reversible=True,
                  assign_mode=_native.EMMA_ASSIGN_BF16X3_RECHECK):
         from .coordinates.covariance import PRECISIONS
 

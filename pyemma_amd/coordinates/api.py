@@ -58,7 +58,7 @@ def _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=True, fo
 def tica(data=None, lag=10, dim=-1, var_cutoff=0.95, kinetic_map=True, commute_map=False, weights="empirical",
          stride=1, remove_mean=True, skip=0, reversible=True, ncov_max=float("inf"), chunksize=None, **kwargs):
     r"""Time-lagged independent component analysis (api.py:1046-1290)."""
-    precision = kwargs.pop("precision", "bf16x4")
+    precision = kwargs.pop("precision", "bf16x4-acc")
     weights = _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=reversible, for_tica=True,
                            precision=precision)
     if not remove_mean:
@@ -116,7 +116,7 @@ def covariance_lagged(data=None, c00=True, c0t=True, ctt=False, remove_constant_
                       reversible=False, bessel=True, lag=0, weights="empirical", stride=1, skip=0, chunksize=None,
                       ncov_max=float("inf"), column_selection=None, diag_only=False, **kwargs):
     r"""Lagged covariances between time series (api.py:1554-1661)."""
-    precision = kwargs.pop("precision", "bf16x4")
+    precision = kwargs.pop("precision", "bf16x4-acc")
     weights = _weights_arg(weights, data, lag, stride, skip, ncov_max, for_tica=False, precision=precision)
     lc = LaggedCovariance(c00=c00, c0t=c0t, ctt=ctt, remove_constant_mean=remove_constant_mean,
                           remove_data_mean=remove_data_mean, reversible=reversible, bessel=bessel, lag=lag,

@@ -38,7 +38,7 @@ class TICA(object):
     _DEFAULT_VARIANCE_CUTOFF = 0.95
 
     def __init__(self, lag, dim=-1, var_cutoff=0.95, kinetic_map=True, commute_map=False, epsilon=1e-6,
-                 stride=1, skip=0, reversible=True, weights=None, ncov_max=float("inf"), precision="bf16x4"):
+                 stride=1, skip=0, reversible=True, weights=None, ncov_max=float("inf"), precision="bf16x4-acc"):
         if kinetic_map and commute_map:
             raise ValueError("Trying to use both kinetic_map and commute_map. Use either or.")
         if (kinetic_map or commute_map) and not reversible:

@@ -672,17 +672,14 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         return (cur.base + (q - cur.p0) * p.stride) * p.d;
     };
 
-    // Frame loads of one stage.  Row pointers are wave-uniform (scalar
-    // registers) and the lane offset a 32-bit VGPR, so the addresses cost no
-    // vector instructions.  Common case: the stage's pairs lie in one
-    // trajectory, the rows are base + q * stride * d and the per-pair lane
-    // offsets are fixed.  Otherwise every pair gets its own row; a pair past
-    // the slice reads the shift vector K for both x and y, which makes its U
-    // and V exactly zero.
-    const int64_t step_d = (int64_t)p.stride * p.d;
-    uint32_t voff[PPT];  // byte offsets of (pair q, column gcol) from the stage's first row
-#pragma unroll
-    for (int q = 0; q < PPT; ++q) voff[q] = (uint32_t)(((int64_t)q * step_d + gcol) * 4);
+    // Frame loads of one stage.  The x and y rows of the stage's pairs are
+    // wave-uniform pointers (scalar registers), found before the stage's
+    // matrix block, and the lane offset is a 32-bit VGPR, so the loads inside
+    // the block are one form for every stage.  Common case: the stage's pairs
+    // lie in one trajectory and the rows are base + q * stride * d.
+    // Otherwise every pair gets its own row; a pair past the slice reads the
+    // shift vector K for both x and y, which makes its U and V exactly zero.
+    const int64_t step_b = (int64_t)p.stride * p.d * 4;
     const uint32_t coff = (uint32_t)gcol * 4u;
     const char *Xb = reinterpret_cast<const char *>(p.X);
     const char *Kb = reinterpret_cast<const char *>(p.K);
@@ -692,24 +689,30 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         return q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi;
     };
     float xa[PPT], ya[PPT], xb[PPT], yb[PPT];
-    auto issue_fast = [&](float *xr, float *yr, int64_t q0) __attribute__((always_inline)) {
-        const int64_t r0 = (cur.base + (q0 - cur.p0) * p.stride) * p.d;
-        const char *bx = Xb + r0 * 4;
-        const char *by = bx + lagd * 4;
+    const char *rx[PPT], *ry[PPT];  // rows of the stage being issued (wave-uniform)
+    auto stage_rows = [&](int t) __attribute__((always_inline)) {
+        const int64_t q0 = stage_q0(t);
+        if (stage_fast(q0)) {
+            const char *bx = Xb + (cur.base + (q0 - cur.p0) * p.stride) * p.d * 4;
 #pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            xr[q] = *reinterpret_cast<const float *>(bx + voff[q]);
-            yr[q] = *reinterpret_cast<const float *>(by + voff[q]);
+            for (int q = 0; q < PPT; ++q) {
+                rx[q] = bx + q * step_b;
+                ry[q] = rx[q] + lagd * 4;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int64_t r = row_of(q0 + q);
+                rx[q] = r >= 0 ? Xb + r * 4 : Kb;
+                ry[q] = r >= 0 ? rx[q] + lagd * 4 : Kb;
+            }
         }
     };
-    auto issue_slow = [&](float *xr, float *yr, int64_t q0) __attribute__((always_inline)) {
+    auto issue_rows = [&](float *xr, float *yr) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
-            const int64_t r = row_of(q0 + q);
-            const char *bx = r >= 0 ? Xb + r * 4 : Kb;
-            const char *by = r >= 0 ? bx + lagd * 4 : Kb;
-            xr[q] = *reinterpret_cast<const float *>(bx + coff);
-            yr[q] = *reinterpret_cast<const float *>(by + coff);
+            xr[q] = *reinterpret_cast<const float *>(rx[q] + coff);
+            yr[q] = *reinterpret_cast<const float *>(ry[q] + coff);
         }
     };
     double sum_u = 0.0;
@@ -844,8 +847,8 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     // then refilled with stage s + 3 ----
     const int n_even = n_stages > 0 ? (n_stages + 1) & ~1 : 2;  // at least one (zero) span: every slot is written
     auto issue = [&](float *xr, float *yr, int t) __attribute__((always_inline)) {
-        const int64_t q0 = stage_q0(t);
-        if (stage_fast(q0)) issue_fast(xr, yr, q0); else issue_slow(xr, yr, q0);
+        stage_rows(t);
+        issue_rows(xr, yr);
     };
     issue(xa, ya, 0);
     issue(xb, yb, 1);
@@ -855,21 +858,17 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     __syncthreads();
     bool first_flush = true;
     // one half-iteration: MFMAs of the stage in buffer mb, preparation of the
-    // register set (xr, yr) into buffer pb, then its refill with stage t; the
-    // two load forms are separate blocks so that each is one schedulable unit
+    // register set (xr, yr) into buffer pb, then its refill with stage t.  The
+    // rows of stage t are found first (scalar code with branches), so that the
+    // matrix block is one basic block, the same for every stage: one copy of
+    // the accumulators stays live across the loop.
     auto half = [&](auto act_tag, int mb, float *xr, float *yr, int pb, int t) __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
-        const int64_t q0 = stage_q0(t);
-        if (stage_fast(q0)) {
-            if constexpr (ACT) mfma(mb);
-            prep(xr, yr, pb);
-            issue_fast(xr, yr, q0);
-            if constexpr (ACT) interleave();
-        } else {
-            if constexpr (ACT) mfma(mb);
-            prep(xr, yr, pb);
-            issue_slow(xr, yr, q0);
-        }
+        stage_rows(t);
+        if constexpr (ACT) mfma(mb);
+        prep(xr, yr, pb);
+        issue_rows(xr, yr);
+        if constexpr (ACT) interleave();
         __syncthreads();
     };
     // waves without matrix work run the same loads, splits and barriers
