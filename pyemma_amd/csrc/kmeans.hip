@@ -67,27 +67,67 @@ __global__ void k_kmpp_first(const float *__restrict__ X, int64_t n, int d, uint
     for (int i = threadIdx.x; i < d; i += blockDim.x) C[i] = X[pick * d + i];
 }
 
-// D^2 update against center c (row of C), block sums; one wavefront per block.
-__global__ __launch_bounds__(256) void k_kmpp_update(const float *__restrict__ X, int64_t n, int d,
-                                                     const float *__restrict__ c, int first,
-                                                     float *__restrict__ d2, double *__restrict__ bsum) {
+// D^2 update against center c (row of C): one thread per frame, the
+// sequential FP32 loop over the dims in order.  VEC: rows 16-byte aligned
+// (d % 4 == 0 and X aligned), read as float4 (the adds stay sequential).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_kmpp_d2(const float *__restrict__ X, int64_t n, int d,
+                                                 const float *__restrict__ c, int first, float *__restrict__ d2) {
     extern __shared__ float s_c[];
     for (int i = threadIdx.x; i < d; i += blockDim.x) s_c[i] = c[i];
     __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + i * d;
+    float di;
+    if constexpr (VEC) {
+        float acc = 0.0f;
+        for (int k = 0; k < d; k += 4) {
+            const float4 xv = *reinterpret_cast<const float4 *>(x + k);
+            const float4 cv = *reinterpret_cast<const float4 *>(s_c + k);
+            float t = __fsub_rn(xv.x, cv.x);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.y, cv.y);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.z, cv.z);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.w, cv.w);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+        }
+        di = acc;
+    } else {
+        di = dist_seq(x, s_c, d);
+    }
+    float v = d2[i];
+    if (first || di < v) v = di;
+    d2[i] = v;
+}
+
+// Block sums of per-frame values (D^2 here, Lloyd's distances below): one
+// wavefront per block of BLOCK frames; lane l sums frames l, l + 64, ... of
+// the block in FP64 in that order, then a butterfly.  The loads of a lane are
+// issued before its adds.  done (may be null): skip once Lloyd has converged.
+__global__ __launch_bounds__(256) void k_block_sums(int64_t n, const float *__restrict__ d2,
+                                                    double *__restrict__ bsum, const int32_t *__restrict__ done) {
+    if (done && *done) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t b = (int64_t)blockIdx.x * 4 + wave;
     const int64_t nb = (n + BLOCK - 1) / BLOCK;
     if (b >= nb) return;
     const int64_t lo = b * BLOCK;
     double s = 0.0;
-    for (int m = 0; m < BLOCK / 64; ++m) {
-        const int64_t i = lo + m * 64 + lane;
-        if (i < n) {
-            const float di = dist_seq(X + i * d, s_c, d);
-            float v = d2[i];
-            if (first || di < v) v = di;
-            d2[i] = v;
-            s = s + (double)v;
+    constexpr int U = 16;
+    for (int m0 = 0; m0 < BLOCK / 64; m0 += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = lo + (m0 + u) * 64 + lane;
+            v[u] = i < n ? d2[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = lo + (m0 + u) * 64 + lane;
+            if (i < n) s = s + (double)v[u];
         }
     }
     s = wave_butterfly_sum(s);
@@ -218,6 +258,7 @@ struct LloydWs {
     int64_t *coff;        // [k + 1] chunk offsets
     int32_t *blk_cnt;     // [nb * k] per-block label counts -> block prefixes
     int64_t *order;       // [n] members sorted by (label, frame)
+    float *dist;          // [n] distance of each frame to its center
     double *chunk_sums;   // [n / CHUNK + k][d]
     void *assign_ws;
     size_t bytes;
@@ -235,6 +276,7 @@ static LloydWs carve_lloyd(void *ws, int64_t n, int d, int k, int mode) {
     w.coff = cv.take<int64_t>((size_t)k + 1);
     w.blk_cnt = cv.take<int32_t>((size_t)nb * k);
     w.order = cv.take<int64_t>(n);
+    w.dist = cv.take<float>(n);
     w.chunk_sums = cv.take<double>((size_t)(n / CHUNK + k + 1) * d);
     w.assign_ws = cv.take<char>(assign_workspace_bytes(n, d, k, mode));
     w.bytes = cv.bytes();
@@ -248,32 +290,42 @@ __global__ void k_lloyd_init(int32_t *done, double *prev_cost, int32_t *n_iter, 
     *converged = 0;
 }
 
-// Per block of 4096 frames (one wavefront): exact distance to the assigned
-// center summed lane-strided + butterfly (the oracle's block sum), and the
-// block's per-label counts.
-__global__ __launch_bounds__(256) void k_lloyd_cost(const float *__restrict__ X, int64_t n, int d,
+// One thread per frame: exact distance to the assigned center (the
+// sequential FP32 loop; VEC as in k_kmpp_d2, with C 16-byte aligned too) and
+// the frame's count in its block's per-label counts.  The block sums of the
+// distances (the oracle's cost order) follow in k_block_sums.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X, int64_t n, int d,
                                                     const float *__restrict__ C, int k,
-                                                    const int32_t *__restrict__ labels, double *__restrict__ bcost,
+                                                    const int32_t *__restrict__ labels, float *__restrict__ dist,
                                                     int32_t *__restrict__ blk_cnt, const int32_t *__restrict__ done) {
     if (*done) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t b = (int64_t)blockIdx.x * 4 + wave;
-    const int64_t nb = (n + BLOCK - 1) / BLOCK;
-    if (b >= nb) return;
-    const int64_t lo = b * BLOCK;
-    int32_t *bc = blk_cnt + b * (int64_t)k;  // zeroed by k_fill_i32 beforehand
-    double s = 0.0;
-    for (int m = 0; m < BLOCK / 64; ++m) {
-        const int64_t i = lo + m * 64 + lane;
-        if (i < n) {
-            const int lab = labels[i];
-            const float di = dist_seq(X + i * d, C + (int64_t)lab * d, d);
-            s = s + (double)di;
-            atomicAdd(&bc[lab], 1);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int lab = labels[i];
+    const float *x = X + i * d;
+    const float *c = C + (int64_t)lab * d;
+    float di;
+    if constexpr (VEC) {
+        float acc = 0.0f;
+        for (int q = 0; q < d; q += 4) {
+            const float4 xv = *reinterpret_cast<const float4 *>(x + q);
+            const float4 cv = *reinterpret_cast<const float4 *>(c + q);
+            float t = __fsub_rn(xv.x, cv.x);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.y, cv.y);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.z, cv.z);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
+            t = __fsub_rn(xv.w, cv.w);
+            acc = __fadd_rn(acc, __fmul_rn(t, t));
         }
+        di = acc;
+    } else {
+        di = dist_seq(x, c, d);
     }
-    s = wave_butterfly_sum(s);
-    if (lane == 0) bcost[b] = s;
+    dist[i] = di;
+    atomicAdd(&blk_cnt[(i / BLOCK) * (int64_t)k + lab], 1);  // zeroed by k_fill_i32 beforehand
 }
 
 // Per label: turn the blk_cnt column into exclusive prefixes over blocks and
@@ -434,9 +486,15 @@ extern "C" int emma_kmeanspp(const float *X, int64_t n, int32_t d, int32_t k, ui
     const int64_t nb = cdiv(n, km::BLOCK);
     km::k_kmpp_first<<<1, 256, 0, s>>>(X, n, d, seed, C_out, idx_out);
     EMMA_TRY(check_launch("kmeans++ first"));
+    const bool vec = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+    const size_t shm = sizeof(float) * (size_t)d;
     for (int r = 1; r < k; ++r) {
-        km::k_kmpp_update<<<(unsigned)cdiv(nb, 4), 256, sizeof(float) * d, s>>>(
-            X, n, d, C_out + (int64_t)(r - 1) * d, r == 1 ? 1 : 0, w.d2, w.bsum);
+        const float *c = C_out + (int64_t)(r - 1) * d;
+        if (vec)
+            km::k_kmpp_d2<true><<<(unsigned)cdiv(n, 256), 256, shm, s>>>(X, n, d, c, r == 1 ? 1 : 0, w.d2);
+        else
+            km::k_kmpp_d2<false><<<(unsigned)cdiv(n, 256), 256, shm, s>>>(X, n, d, c, r == 1 ? 1 : 0, w.d2);
+        km::k_block_sums<<<(unsigned)cdiv(nb, 4), 256, 0, s>>>(n, w.d2, w.bsum, nullptr);
         km::k_kmpp_pick<<<1, 64, 0, s>>>(X, n, d, seed, r, w.d2, w.bsum, w.sbsum, C_out, idx_out);
     }
     return check_launch("kmeans++ rounds");
@@ -463,10 +521,18 @@ extern "C" int emma_lloyd(const float *X, int64_t n, int32_t d, float *C, int32_
     hipStream_t s = (hipStream_t)stream;
     const int64_t nb = cdiv(n, km::BLOCK);
     km::k_lloyd_init<<<1, 1, 0, s>>>(w.done, w.prev_cost, n_iter, converged);
+    const bool vec = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(C) & 15) == 0;
     for (int it = 0; it < max_iter; ++it) {
         EMMA_TRY(assign_run(X, n, d, C, k, labels, assign_mode, nullptr, w.assign_ws, w.done, s));
         km::k_fill_i32<<<1024, 256, 0, s>>>(w.blk_cnt, nb * (int64_t)k, w.done);
-        km::k_lloyd_cost<<<(unsigned)cdiv(nb, 4), 256, 0, s>>>(X, n, d, C, k, labels, w.bcost, w.blk_cnt, w.done);
+        if (vec)
+            km::k_lloyd_dist<true><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist, w.blk_cnt,
+                                                                        w.done);
+        else
+            km::k_lloyd_dist<false><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist, w.blk_cnt,
+                                                                         w.done);
+        km::k_block_sums<<<(unsigned)cdiv(nb, 4), 256, 0, s>>>(n, w.dist, w.bcost, w.done);
         km::k_lloyd_scan<<<1, 1024, 0, s>>>(k, nb, w.cnt, w.off, w.coff, w.blk_cnt, w.done);
         km::k_lloyd_scatter<<<(unsigned)nb, 64, sizeof(int32_t) * k, s>>>(n, k, labels, w.off, w.blk_cnt, w.order,
                                                                           w.done);
