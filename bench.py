@@ -261,8 +261,8 @@ def adjacent_legs(args, X, off_dev, offsets, labels, xyz):
                      "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "pairs_counted": total, "pairs_expected": expect, "bad": int(bad.item())}
     ms = _timed(lambda: _native.moments_lagged(X, off_dev, args.lag, reversible=False, remove_mean=True, c0t=True,
-                                               ctt=True, precision=_native.EMMA_PREC_BF16X4), reps=1)
-    out["vamp_moments"] = {"ms": ms, "precision": "bf16x4", "matrices": "C00, C0t, Ctt (separate means)"}
+                                               ctt=True, precision=_native.EMMA_PREC_BF16X4_ACC), reps=1)
+    out["vamp_moments"] = {"ms": ms, "precision": "bf16x4-acc", "matrices": "C00, C0t, Ctt (separate means)"}
     log("featurize %.2f ms, counts %.2f ms, vamp moments %.2f ms"
         % (out["featurize"]["ms"], out["counts"]["ms"], out["vamp_moments"]["ms"]))
     torch.cuda.synchronize()

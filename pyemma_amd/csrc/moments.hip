@@ -596,9 +596,19 @@ struct PairCursor {
     int64_t p0, p1, base;
 };
 
-template <bool DIAG, bool TL>
-__device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&s_sum)[4][RTB], int I, int J,
-                                         int task, int slice) {
+// MODE 0 (REV): UU and VV as above.  MODE 1 (NRS): the non-reversible
+// symmetric matrices instead (XX from x' = x - K in place of U, YY from
+// y' = y - K in place of V; column sums of x' and y'), the C00 / Ctt part of
+// the non-reversible and VAMP moments; without the Ctt flag the YY waves only
+// load and split.  MODE 2 (XY): the lagged products x'y'^T of the tasks
+// (I, J) and (J, I) from the same loads: waves 0-3 compute x'_I^T y'_J, waves
+// 4-7 x'_J^T y'_I (on a diagonal tile only waves 0-3, the whole tile), each
+// into the slot of its own non-symmetric task; no column sums.
+template <bool DIAG, bool TL, int MODE>
+__device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&s_sum)[2][4][RTB], int I,
+                                         int J, int task, int slice) {
+    constexpr bool NRS = MODE != 0;  // operands x', y' (not U, V)
+    constexpr bool XY = MODE == 2;
     constexpr int PPT = DIAG ? 4 : 8;  // pairs per loader thread per stage
     constexpr int TB = RTB;
 
@@ -630,7 +640,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
 
     // ---- MFMA role ----
     int m, wr, wc;
-    if (DIAG) {
+    if (DIAG && !XY) {
         m = wave & 1;
         wr = wave >> 2;
         wc = (wave >> 1) & 1;
@@ -639,9 +649,22 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         wr = (wave >> 1) & 1;
         wc = wave & 1;
     }
-    const bool wave_act = (I * TB + wr * 64 < p.d) && (J * TB + wc * 64 < p.d) && !(DIAG && wr > wc);
-    const int sa = m == 0 ? 0 : 2;
-    const int sb = DIAG ? sa : sa + 1;
+    bool wave_act;
+    int sa, sb;
+    double *mslot = slot + (int64_t)m * TB * TB;  // output matrix of this wave
+    if constexpr (XY) {
+        const int ri = m == 0 ? I : J, ci = m == 0 ? J : I;  // row (x') and column (y') blocks
+        wave_act = (ri * TB + wr * 64 < p.d) && (ci * TB + wc * 64 < p.d) && !(DIAG && m == 1);
+        sa = DIAG ? 0 : (m == 0 ? 0 : 1);  // x' of the row block
+        sb = DIAG ? 2 : (m == 0 ? 3 : 2);  // y' of the column block
+        const int txy = p.n_sym + ri * p.nb + ci;
+        mslot = p.slots + ((int64_t)slice * p.n_tasks + txy) * p.slot_doubles;
+    } else {
+        wave_act = (I * TB + wr * 64 < p.d) && (J * TB + wc * 64 < p.d) && !(DIAG && wr > wc) &&
+                   !(NRS && m == 1 && !p.ctt);
+        sa = m == 0 ? 0 : 2;
+        sb = DIAG ? sa : sa + 1;
+    }
 
     // ---- cursor: trajectory of the slice's first pair ----
     // (the tables are read-only here: uniform reads go through the scalar cache)
@@ -715,21 +738,33 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
             yr[q] = *reinterpret_cast<const float *>(ry[q] + coff);
         }
     };
-    double sum_u = 0.0;
+    double sum_u = 0.0, sum_v = 0.0;
     auto prep = [&](const float *xr, const float *yr, int buf) __attribute__((always_inline)) {
         uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
-        float bu = 0.0f;
+        float bu = 0.0f, bv = 0.0f;
 #pragma unroll
         for (int q = 0; q < PPT; q += 2) {
-            const float u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
-            const float u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
-            const float v0 = __fsub_rn(xr[q], yr[q]);
-            const float v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
+            float u0, u1, v0, v1;
+            if constexpr (NRS) {
+                u0 = __fsub_rn(xr[q], Kc);
+                u1 = __fsub_rn(xr[q + 1], Kc);
+                v0 = __fsub_rn(yr[q], Kc);
+                v1 = __fsub_rn(yr[q + 1], Kc);
+                if constexpr (DIAG) bv = __fadd_rn(bv, __fadd_rn(v0, v1));
+            } else {
+                u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
+                u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
+                v0 = __fsub_rn(xr[q], yr[q]);
+                v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
+            }
             if constexpr (DIAG) bu = __fadd_rn(bu, __fadd_rn(u0, u1));
             split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
             split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
         }
-        if constexpr (DIAG) sum_u += (double)bu;
+        if constexpr (DIAG) {
+            sum_u += (double)bu;
+            if constexpr (NRS) sum_v += (double)bv;
+        }
         const int off = lcol * RPITCH + pfirst;
         if constexpr (PPT == 8) {
             *reinterpret_cast<uint4 *>(&lds.op[buf][su][0][off]) = make_uint4(uh[0], uh[1], uh[2], uh[3]);
@@ -824,7 +859,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
                 // element-major fragment order: one instruction covers whole cache
                 // lines; wave-uniform base + 32-bit lane offset (no 64-bit vector
                 // addresses to keep live across the loop)
-                char *base = reinterpret_cast<char *>(slot + (int64_t)m * TB * TB +
+                char *base = reinterpret_cast<char *>(mslot +
                                                       (int64_t)((wr * 2 + wc) * 4 + r * 2 + c) * 1024);
                 uint32_t loff = (uint32_t)lane * 8u;
                 asm volatile("" : "+v"(loff));  // recomputed per flush: keeps the addresses out of the loop
@@ -895,28 +930,32 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     else
         run(std::integral_constant<bool, false>());
 
-    // ---- column sums of U (= shifted x + shifted y), diagonal tasks only ----
-    if (DIAG) {
-        s_sum[grp][lcol] = sum_u;
+    // ---- column sums, diagonal tasks only: of U (= shifted x + shifted y) and
+    // zero, or (NRS) of x' and y' ----
+    if (DIAG && !XY) {
+        s_sum[0][grp][lcol] = sum_u;
+        s_sum[1][grp][lcol] = sum_v;
         __syncthreads();
         if (tid < TB) {
-            const double a = s_sum[0][tid] + s_sum[1][tid] + s_sum[2][tid] + s_sum[3][tid];
+            const double a = s_sum[0][0][tid] + s_sum[0][1][tid] + s_sum[0][2][tid] + s_sum[0][3][tid];
+            const double b = s_sum[1][0][tid] + s_sum[1][1][tid] + s_sum[1][2][tid] + s_sum[1][3][tid];
             double *sums = slot + 2LL * TB * TB;
             sums[tid] = a;
-            sums[TB + tid] = 0.0;
+            sums[TB + tid] = NRS ? b : 0.0;
         }
     }
 }
 
-// All tasks of the reversible bf16 moments in one launch.  The workgroups of
-// one pair slice are consecutive in one XCD's dispatch order (off-diagonal
-// tasks first, then the diagonal ones), so the tasks that read the same frame
-// rows run side by side and share the XCD's L2: each frame comes from HBM
-// about once per call instead of once per kind of task.
-template <bool TL>
+// All tasks of the reversible bf16 moments (NRS: the symmetric tasks of the
+// non-reversible ones) in one launch.  The workgroups of one pair slice are
+// consecutive in one XCD's dispatch order (off-diagonal tasks first, then the
+// diagonal ones), so the tasks that read the same frame rows run side by side
+// and share the XCD's L2: each frame comes from HBM about once per call
+// instead of once per kind of task.
+template <bool TL, int MODE>
 __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     __shared__ __align__(16) RevLds lds;
-    __shared__ double s_sum[4][RTB];
+    __shared__ double s_sum[2][4][RTB];
     const int per_xcd = gridDim.x >> 3;
     const int wg = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (wg >= p.n_sym * p.n_slices) return;
@@ -939,9 +978,9 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     for (int i = 0; i < I; ++i) task += p.nb - i;
     task += J - I;
     if (I == J)
-        rev_body<true, TL>(p, lds, s_sum, I, J, task, slice);
+        rev_body<true, TL, MODE>(p, lds, s_sum, I, J, task, slice);
     else
-        rev_body<false, TL>(p, lds, s_sum, I, J, task, slice);
+        rev_body<false, TL, MODE>(p, lds, s_sum, I, J, task, slice);
 }
 
 // ------------------------------------------------------------------
@@ -1058,10 +1097,12 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
         if (Myy) Myy[e] = 0.0;
         return;
     }
+    // the correction terms are grouped so that (i, j) and (j, i) round alike:
+    // the symmetric matrices come out exactly symmetric
     if (p.lag == 0) {
         const double raw = slot_sum(p, ts, 0, lis, ljs);
         const double w = Wp;
-        Mxx[e] = remove_mean ? raw - Sx[i] * Sx[j] / w : raw + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
+        Mxx[e] = remove_mean ? raw - Sx[i] * Sx[j] / w : raw + ((Sx[i] * K_j + K_i * Sx[j]) + w * (K_i * K_j));
         return;
     }
     if (p.rev) {
@@ -1070,22 +1111,22 @@ __global__ __launch_bounds__(256) void k_reduce_mats(Params p, const double *Sx,
         const double w = 2.0 * Wp;
         const double Si = Sx[i] + Sy[i], Sj = Sx[j] + Sy[j];
         const double rxx = 0.5 * (uu + vv), rxy = 0.5 * (uu - vv);
-        const double corr = remove_mean ? -Si * Sj / w : Si * K_j + K_i * Sj + w * K_i * K_j;
+        const double corr = remove_mean ? -(Si * Sj) / w : (Si * K_j + K_i * Sj) + w * (K_i * K_j);
         Mxx[e] = rxx + corr;
         if (Mxy) Mxy[e] = rxy + corr;
         return;
     }
     const double w = Wp;
     const double rxx = slot_sum(p, ts, 0, lis, ljs);
-    Mxx[e] = remove_mean ? rxx - Sx[i] * Sx[j] / w : rxx + Sx[i] * K_j + K_i * Sx[j] + w * K_i * K_j;
+    Mxx[e] = remove_mean ? rxx - Sx[i] * Sx[j] / w : rxx + ((Sx[i] * K_j + K_i * Sx[j]) + w * (K_i * K_j));
     if (Myy) {
         const double ryy = slot_sum(p, ts, 1, lis, ljs);
-        Myy[e] = remove_mean ? ryy - Sy[i] * Sy[j] / w : ryy + Sy[i] * K_j + K_i * Sy[j] + w * K_i * K_j;
+        Myy[e] = remove_mean ? ryy - Sy[i] * Sy[j] / w : ryy + ((Sy[i] * K_j + K_i * Sy[j]) + w * (K_i * K_j));
     }
     if (Mxy) {
         const int tx = p.n_sym + I * p.nb + J;
         const double rxy = slot_sum(p, tx, 0, li, lj);
-        Mxy[e] = remove_mean ? rxy - Sx[i] * Sy[j] / w : rxy + Sx[i] * K_j + K_i * Sy[j] + w * K_i * K_j;
+        Mxy[e] = remove_mean ? rxy - Sx[i] * Sy[j] / w : rxy + ((Sx[i] * K_j + K_i * Sy[j]) + w * (K_i * K_j));
     }
 }
 
@@ -1096,9 +1137,21 @@ static void launch_kind_w(const Params &p, hipStream_t s) {
     if constexpr (PREC == EMMA_PREC_BF16X4 && !WT) {
         if (p.rev) {
             if (p.acc2)
-                k_rev_bf16<true><<<g_sym, RNT, 0, s>>>(p);
+                k_rev_bf16<true, 0><<<g_sym, RNT, 0, s>>>(p);
             else
-                k_rev_bf16<false><<<g_sym, RNT, 0, s>>>(p);
+                k_rev_bf16<false, 0><<<g_sym, RNT, 0, s>>>(p);
+            return;
+        }
+        if (p.lag > 0) {
+            // C00 (and Ctt), then C0t: two launches of the pipelined kernel
+            // over the I <= J tiles
+            if (p.acc2) {
+                k_rev_bf16<true, 1><<<g_sym, RNT, 0, s>>>(p);
+                if (p.has_xy) k_rev_bf16<true, 2><<<g_sym, RNT, 0, s>>>(p);
+            } else {
+                k_rev_bf16<false, 1><<<g_sym, RNT, 0, s>>>(p);
+                if (p.has_xy) k_rev_bf16<false, 2><<<g_sym, RNT, 0, s>>>(p);
+            }
             return;
         }
     }
@@ -1163,7 +1216,7 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
     const bool weighted = (flags & EMMA_MOM_WEIGHTED) != 0;
-    if (precision == EMMA_PREC_BF16X4 && p.rev && !weighted) {
+    if (precision == EMMA_PREC_BF16X4 && lag > 0 && !weighted) {
         // one 8-wave workgroup per CU: 256 slices make the launch of
         // k_rev_bf16 a whole number of rounds (one per task) over the 256 CUs
         const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
@@ -1201,8 +1254,7 @@ namespace mom {
 static int effective_precision(int precision, uint32_t flags, int lag, int *acc2) {
     *acc2 = 0;
     if (precision != EMMA_PREC_BF16X4_ACC) return precision;
-    const bool rev = (flags & EMMA_MOM_SYMMETRIZE) && lag > 0;
-    if (rev && !(flags & EMMA_MOM_WEIGHTED)) {
+    if (lag > 0 && !(flags & EMMA_MOM_WEIGHTED)) {
         *acc2 = 1;
         return EMMA_PREC_BF16X4;
     }

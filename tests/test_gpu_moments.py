@@ -54,20 +54,26 @@ def test_reversible_mean_free(prec, d):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("remove_mean", [True, False])
-def test_nonreversible_with_ctt(prec, remove_mean):
-    rng = np.random.default_rng(4)
-    trajs = [(rng.standard_normal((T, 7)).cumsum(0) * 0.2 + 1.0).astype(np.float32) for T in (800, 333, 5)]
+@pytest.mark.parametrize("d,ctt", [(7, True), (130, True), (300, True), (300, False)])
+def test_nonreversible_with_ctt(prec, remove_mean, d, ctt):
+    rng = np.random.default_rng(4 + d)
+    trajs = [(rng.standard_normal((T, d)).cumsum(0) * 0.2 + 1.0).astype(np.float32) for T in (800, 333, 5)]
     lag = 3
     ref = om.covariance_lagged(trajs, lag, c00=True, c0t=True, ctt=True, remove_data_mean=remove_mean,
                                reversible=False, bessel=False)
     w, sx, sy, Mxx, Mxy, Myy = _gpu(trajs, lag, prec, reversible=False, remove_mean=remove_mean, c0t=True,
-                                   ctt=True)
+                                   ctt=ctt)
     assert w[0] == ref["w"]
     _close(sx / w[0], ref["mean"], rtol=1e-6, prec=prec)
     _close(sy / w[0], ref["mean_tau"], rtol=1e-6, prec=prec)
     _close(Mxx / w[0], ref["C00"], prec=prec)
     _close(Mxy / w[0], ref["C0t"], prec=prec)
-    _close(Myy / w[0], ref["Ctt"], prec=prec)
+    if ctt:
+        _close(Myy / w[0], ref["Ctt"], prec=prec)
+        np.testing.assert_array_equal(Myy, Myy.T)
+    else:
+        assert Myy is None
+    np.testing.assert_array_equal(Mxx, Mxx.T)
 
 
 @pytest.mark.parametrize("prec", PRECS)

@@ -59,7 +59,7 @@ def test_vamp_cross_validation_score(msm_vamp):
                                    vt.model.score(ve.model, score_method=method), rtol=1e-12)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x4"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x4", "bf16x4-acc"])
 def test_vamp_covariances_vs_oracle(precision):
     rng = np.random.RandomState(3)
     trajs = [np.cumsum(rng.randn(n, 12), axis=0).astype(np.float32) * 0.1 + rng.randn(12).astype(np.float32)
