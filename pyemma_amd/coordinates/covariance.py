@@ -96,11 +96,10 @@ class LaggedCovariance(object):
     (FP32 MFMA) accumulate tiles in FP32 over at most 1024 pairs and merge in
     FP64; 'fp64' runs FP64 MFMA with FP64 accumulation (agrees with the FP64
     reference computation to rounding, also for near-zero eigenvalues).
-    'bf16x4-acc' (the default) runs the lagged, unweighted moments (reversible
-    or not) as 'bf16x4' with two-level FP32 accumulation (sub-spans of 64
-    pairs added into the 1024-pair span sum; TICA eigenvalues within 1e-6
-    relative of FP64 on 300 collinear distance features) and the weighted and
-    lag-0 kinds as 'fp64'.
+    'bf16x4-acc' (the default) runs the unweighted moments as 'bf16x4' with
+    two-level FP32 accumulation (sub-spans of 64 pairs added into the
+    1024-pair span sum; TICA eigenvalues within 1e-6 relative of FP64 on 300
+    collinear distance features) and the weighted ones as 'fp64'.
     """
 
     def __init__(self, c00=True, c0t=False, ctt=False, remove_constant_mean=None, remove_data_mean=False,

@@ -1142,7 +1142,15 @@ static void launch_kind_w(const Params &p, hipStream_t s) {
                 k_rev_bf16<false, 0><<<g_sym, RNT, 0, s>>>(p);
             return;
         }
-        if (p.lag > 0) {
+        if (p.lag == 0) {
+            // XX only: mode NRS with y = x (the YY waves only load and split)
+            if (p.acc2)
+                k_rev_bf16<true, 1><<<g_sym, RNT, 0, s>>>(p);
+            else
+                k_rev_bf16<false, 1><<<g_sym, RNT, 0, s>>>(p);
+            return;
+        }
+        {
             // C00 (and Ctt), then C0t: two launches of the pipelined kernel
             // over the I <= J tiles
             if (p.acc2) {
@@ -1216,7 +1224,7 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
     const bool weighted = (flags & EMMA_MOM_WEIGHTED) != 0;
-    if (precision == EMMA_PREC_BF16X4 && lag > 0 && !weighted) {
+    if (precision == EMMA_PREC_BF16X4 && !weighted) {
         // one 8-wave workgroup per CU: 256 slices make the launch of
         // k_rev_bf16 a whole number of rounds (one per task) over the 256 CUs
         const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
@@ -1254,7 +1262,7 @@ namespace mom {
 static int effective_precision(int precision, uint32_t flags, int lag, int *acc2) {
     *acc2 = 0;
     if (precision != EMMA_PREC_BF16X4_ACC) return precision;
-    if (lag > 0 && !(flags & EMMA_MOM_WEIGHTED)) {
+    if (!(flags & EMMA_MOM_WEIGHTED)) {
         *acc2 = 1;
         return EMMA_PREC_BF16X4;
     }

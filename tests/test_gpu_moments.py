@@ -77,14 +77,17 @@ def test_nonreversible_with_ctt(prec, remove_mean, d, ctt):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_lag0_stride_skip(prec):
+@pytest.mark.parametrize("d", [5, 300])
+def test_lag0_stride_skip(prec, d):
     rng = np.random.default_rng(6)
-    trajs = [(rng.standard_normal((T, 5)) + 2).astype(np.float32) for T in (100, 37)]
+    trajs = [(rng.standard_normal((T, d)) + 2).astype(np.float32) for T in (100, 37)]
     ref = om.covariance_lagged(trajs, 0, c00=True, c0t=False, remove_data_mean=True, bessel=False,
                                stride=3, skip=2)
     w, sx, _, Mxx, _, _ = _gpu(trajs, 0, prec, stride=3, skip=2, reversible=False, remove_mean=True, c0t=False)
     assert w[0] == ref["w"]
+    _close(sx / w[0], ref["mean"], rtol=1e-6, prec=prec)
     _close(Mxx / w[0], ref["C00"], prec=prec)
+    np.testing.assert_array_equal(Mxx, Mxx.T)
     for lag, stride, skip in [(4, 3, 2), (1, 2, 5)]:
         ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=True, bessel=False,
                                    stride=stride, skip=skip)
