@@ -96,10 +96,9 @@ class LaggedCovariance(object):
     (FP32 MFMA) accumulate tiles in FP32 over at most 1024 pairs and merge in
     FP64; 'fp64' runs FP64 MFMA with FP64 accumulation (agrees with the FP64
     reference computation to rounding, also for near-zero eigenvalues).
-    'bf16x4-acc' (the default) runs the unweighted moments as 'bf16x4' with
-    two-level FP32 accumulation (sub-spans of 64 pairs added into the
-    1024-pair span sum; TICA eigenvalues within 1e-6 relative of FP64 on 300
-    collinear distance features) and the weighted ones as 'fp64'.
+    'bf16x4-acc' (the default) runs 'bf16x4' with two-level FP32 accumulation
+    (sub-spans of 64 pairs added into the 1024-pair span sum; TICA eigenvalues
+    within 1e-6 relative of FP64 on 300 collinear distance features).
     """
 
     def __init__(self, c00=True, c0t=False, ctt=False, remove_constant_mean=None, remove_data_mean=False,
@@ -153,7 +152,8 @@ class LaggedCovariance(object):
         lag = self.lag
         wts = frame_weights(self.weights, dt)
         if wts is not None and self.ctt:
-            raise NotImplementedError("weighted Ctt is not supported")
+            # running_moments.py:255-256
+            raise ValueError("Use of weights is not implemented for compute_YY==True")
         w, sx, sy, Mxx, Mxy, Myy = _native.moments_lagged(
             dt.X, dt.offsets_dev, lag, stride=self.stride, skip=self.skip, reversible=self.reversible,
             remove_mean=self.remove_data_mean, c0t=self.c0t, ctt=self.ctt,

@@ -14,17 +14,19 @@
 //   sums are S + w K.
 //
 // Parallel layout: the d x d outputs are cut into TB x TB tiles (tasks); the
-// pair index space is cut into slices.  One workgroup = (task, slice).  Per
-// stage of BT = 16 pairs the workgroup stages the operand columns of its two
-// column blocks in LDS, then 4 waves (2 x 2) run MFMA tiles.  The frame loads
-// of stage s+1 are in flight while stage s runs on the matrix cores.
+// pair index space is cut into slices.  One workgroup = (task, slice).
 //
-// Precisions:
-//   BF16X4  operands split into bf16 hi + lo, products hh, hl, lh, ll on
-//           v_mfma_f32_32x32x16_bf16, FP32 tile accumulation, TB = 128
-//   FP32    v_mfma_f32_32x32x2_f32, FP32 tile accumulation, TB = 128
-//   FP64    v_mfma_f64_16x16x4_f64 on FP64 operands (x - K exact in FP64),
-//           FP64 accumulation, TB = 64
+// Precisions and kernels:
+//   BF16X4  k_rev_bf16 (every kind of moments): operands split into bf16 hi +
+//           lo, products hh, hl, lh, ll on v_mfma_f32_32x32x16_bf16, FP32 tile
+//           accumulation (two-level for BF16X4_ACC), TB = 128, 8 waves, two
+//           LDS buffers, frame loads two stages ahead
+//   FP32    k_main: v_mfma_f32_32x32x2_f32, FP32 tile accumulation, TB = 128
+//   FP64    k_main: v_mfma_f64_16x16x4_f64 on FP64 operands (x - K exact in
+//           FP64), FP64 accumulation, TB = 64
+// k_main stages the operand columns of its two column blocks in LDS per stage
+// of BT = 16 pairs, then 4 waves (2 x 2) run MFMA tiles; the frame loads of
+// stage s+1 are in flight while stage s runs on the matrix cores.
 // FP32 accumulators are flushed every FLUSH pairs into the workgroup's private
 // FP64 slot in the workspace; a final kernel sums the slots in slice order
 // (deterministic) and applies the FP64 corrections.
@@ -51,7 +53,7 @@ struct Params {
     int ntraj;
     int lag, stride, skip;
     int tb;               // tile edge
-    int acc2;             // reversible bf16 kernel: two-level FP32 accumulation
+    int acc2;             // bf16 kernel: two-level FP32 accumulation
     int frag;             // slot layout: 1 = MFMA fragment order (TB 128), 0 = row-major
     int64_t slot_doubles; // doubles per slot: 2 matrices + 2 column-sum vectors
     int nb;               // column blocks = ceil(d / tb)
@@ -67,6 +69,7 @@ struct Params {
     float *K;             // [d]
     const double *wts;    // per-row frame weights (nullptr: unweighted)
     double *wpart;        // [ntraj] weight sum of each trajectory's pairs
+    float *wpair;         // [P] weight of each pair as FP32, in pair order (weighted)
     double *W;            // [1] total pair weight
     double *slots;        // n_tasks * n_slices * slot_doubles
 };
@@ -144,13 +147,6 @@ __global__ __launch_bounds__(1024) void k_prep(Params p) {
 // ------------------------------------------------------------------
 template <int PREC>
 struct Cfg;
-template <>
-struct Cfg<EMMA_PREC_BF16X4> {
-    static constexpr int TB = 128;
-    static constexpr int PITCH = BT + 8;  // uint16 per column (48 B: conflict-free 16-byte reads)
-    typedef uint16_t elem;
-    static constexpr int PARTS = 2;       // hi, lo
-};
 template <>
 struct Cfg<EMMA_PREC_FP32> {
     static constexpr int TB = 128;
@@ -331,34 +327,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
         }
         elem *dst1 = &s_op[sl1][0][lcol * PITCH + pq * PPT];
         elem *dst2 = &s_op[sl2][0][lcol * PITCH + pq * PPT];
-        if constexpr (PREC == EMMA_PREC_BF16X4) {
-#pragma unroll
-            for (int g = 0; g < PPT; g += 8) {
-                uint32_t h[4], l[4];
-#pragma unroll
-                for (int q = 0; q < 8; q += 2) {
-                    uint32_t ha, la, hb, lb;
-                    split_bf16(o1[g + q], ha, la);
-                    split_bf16(o1[g + q + 1], hb, lb);
-                    h[q / 2] = pack2(ha, hb);
-                    l[q / 2] = pack2(la, lb);
-                }
-                *reinterpret_cast<uint4 *>(dst1 + g) = make_uint4(h[0], h[1], h[2], h[3]);
-                *reinterpret_cast<uint4 *>(dst1 + TB * PITCH + g) = make_uint4(l[0], l[1], l[2], l[3]);
-                if constexpr (NMAT == 2) {
-#pragma unroll
-                    for (int q = 0; q < 8; q += 2) {
-                        uint32_t ha, la, hb, lb;
-                        split_bf16(o2[g + q], ha, la);
-                        split_bf16(o2[g + q + 1], hb, lb);
-                        h[q / 2] = pack2(ha, hb);
-                        l[q / 2] = pack2(la, lb);
-                    }
-                    *reinterpret_cast<uint4 *>(dst2 + g) = make_uint4(h[0], h[1], h[2], h[3]);
-                    *reinterpret_cast<uint4 *>(dst2 + TB * PITCH + g) = make_uint4(l[0], l[1], l[2], l[3]);
-                }
-            }
-        } else if constexpr (PREC == EMMA_PREC_FP32) {
+        if constexpr (PREC == EMMA_PREC_FP32) {
 #pragma unroll
             for (int q = 0; q < PPT; ++q) dst1[q] = o1[q];
             if constexpr (NMAT == 2) {
@@ -383,36 +352,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 #pragma unroll
         for (int m = 0; m < NMAT; ++m) {
             const int sa = m == 0 ? 0 : 2, sb = m == 0 ? 1 : 3;
-            if constexpr (PREC == EMMA_PREC_BF16X4) {
-                bf16x8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    const int off = (wr * 64 + r * 32 + (lane & 31)) * PITCH + (lane >> 5) * 8;
-                    ah[r] = *reinterpret_cast<const bf16x8 *>(&s_op[sa][0][off]);
-                    al[r] = *reinterpret_cast<const bf16x8 *>(&s_op[sa][0][TB * PITCH + off]);
-                }
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int off = (wc * 64 + c * 32 + (lane & 31)) * PITCH + (lane >> 5) * 8;
-                    bh[c] = *reinterpret_cast<const bf16x8 *>(&s_op[sb][0][off]);
-                    bl[c] = *reinterpret_cast<const bf16x8 *>(&s_op[sb][0][TB * PITCH + off]);
-                }
-#pragma unroll
-                for (int r = 0; r < 2; ++r)
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        // small products first
-                        if constexpr (FIRST) {
-                            const floatx16 z = {};
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], z, 0, 0, 0);
-                        } else {
-                            acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], acc[m][r][c], 0, 0, 0);
-                        }
-                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[m][r][c], 0, 0, 0);
-                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[m][r][c], 0, 0, 0);
-                        acc[m][r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[m][r][c], 0, 0, 0);
-                    }
-            } else if constexpr (PREC == EMMA_PREC_FP32) {
+            if constexpr (PREC == EMMA_PREC_FP32) {
 #pragma unroll
                 for (int kk = 0; kk < BT; kk += 2) {
                     float av[2], bv[2];
@@ -560,7 +500,7 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 }
 
 // ------------------------------------------------------------------
-// reversible moments, bf16 hi/lo split: the pipelined TICA kernel
+// bf16 hi/lo split moments: the pipelined kernel (TICA, VAMP, PCA, weighted)
 // ------------------------------------------------------------------
 // One workgroup = (task, slice) as above, 8 waves.  Per stage of RBT = 16
 // pairs the loader threads read x and y of their columns, form U = x' + y',
@@ -604,7 +544,12 @@ struct PairCursor {
 // (I, J) and (J, I) from the same loads: waves 0-3 compute x'_I^T y'_J, waves
 // 4-7 x'_J^T y'_I (on a diagonal tile only waves 0-3, the whole tile), each
 // into the slot of its own non-symmetric task; no column sums.
-template <bool DIAG, bool TL, int MODE>
+// WT: frame weights (the weight of a pair's x frame) multiply the A operand
+// and the column sums: off-diagonal tiles weight the block-I operands; a
+// diagonal tile writes weighted copies of U and V (x' and y') into the LDS
+// slots it does not otherwise use and takes A from them; mode XY weights x'
+// (the A operand of both of its tasks) and never y'.
+template <bool DIAG, bool TL, int MODE, bool WT>
 __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&s_sum)[2][4][RTB], int I,
                                          int J, int task, int slice) {
     constexpr bool NRS = MODE != 0;  // operands x', y' (not U, V)
@@ -664,6 +609,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
                    !(NRS && m == 1 && !p.ctt);
         sa = m == 0 ? 0 : 2;
         sb = DIAG ? sa : sa + 1;
+        if (WT && DIAG) sa += 1;  // the weighted copy
     }
 
     // ---- cursor: trajectory of the slice's first pair ----
@@ -683,7 +629,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         cur.p1 = p.pair_off[lo + 1];
         cur.base = p.traj_off[lo] + p.skip;
     }
-    // element offset of the x row of pair q (wave-uniform q; -1 if q is past the slice)
+    // x row (frame index) of pair q (wave-uniform q; -1 if q is past the slice)
     auto row_of = [&](int64_t q) __attribute__((always_inline)) -> int64_t {
         if (q >= p_hi) return -1;
         while (q >= cur.p1) {
@@ -692,7 +638,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
             cur.p1 = poff_c[cur.t + 1];
             cur.base = toff_c[cur.t] + p.skip;
         }
-        return (cur.base + (q - cur.p0) * p.stride) * p.d;
+        return cur.base + (q - cur.p0) * p.stride;
     };
 
     // Frame loads of one stage.  The x and y rows of the stage's pairs are
@@ -712,9 +658,16 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         return q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi;
     };
     float xa[PPT], ya[PPT], xb[PPT], yb[PPT];
+    // WT: lane l of a register set's weight holds the weight of the set's
+    // pair l (l < PPT; pairs are consecutive in the pair-ordered weights), so
+    // one vector load per stage brings them with the frames; prep reads them
+    // back with readlane
+    float wa[1], wb[1];
     const char *rx[PPT], *ry[PPT];  // rows of the stage being issued (wave-uniform)
+    int64_t q0n = 0;                // first pair of the stage being issued
     auto stage_rows = [&](int t) __attribute__((always_inline)) {
         const int64_t q0 = stage_q0(t);
+        q0n = q0;
         if (stage_fast(q0)) {
             const char *bx = Xb + (cur.base + (q0 - cur.p0) * p.stride) * p.d * 4;
 #pragma unroll
@@ -726,21 +679,30 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int64_t r = row_of(q0 + q);
-                rx[q] = r >= 0 ? Xb + r * 4 : Kb;
+                rx[q] = r >= 0 ? Xb + r * p.d * 4 : Kb;
                 ry[q] = r >= 0 ? rx[q] + lagd * 4 : Kb;
             }
         }
     };
-    auto issue_rows = [&](float *xr, float *yr) __attribute__((always_inline)) {
+    auto issue_rows = [&](float *xr, float *yr, float *wr) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             xr[q] = *reinterpret_cast<const float *>(rx[q] + coff);
             yr[q] = *reinterpret_cast<const float *>(ry[q] + coff);
         }
+        if constexpr (WT) {
+            // pairs past the slice (U = V = 0) read any valid weight
+            const int64_t qw = q0n + (lane & (PPT - 1));
+            wr[0] = P > 0 ? p.wpair[qw < P ? qw : P - 1] : 0.0f;
+        }
     };
     double sum_u = 0.0, sum_v = 0.0;
-    auto prep = [&](const float *xr, const float *yr, int buf) __attribute__((always_inline)) {
+    // weighted A operand of this loader thread (WT, off-diagonal modes REV / NRS:
+    // block I; mode XY: every x')
+    const bool wscale = WT && !(DIAG && !XY) && (XY || su == 0);
+    auto prep = [&](const float *xr, const float *yr, const float *wr, int buf) __attribute__((always_inline)) {
         uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
+        uint32_t uwh[PPT / 2], uwl[PPT / 2], vwh[PPT / 2], vwl[PPT / 2];  // weighted copies (WT, diagonal)
         float bu = 0.0f, bv = 0.0f;
 #pragma unroll
         for (int q = 0; q < PPT; q += 2) {
@@ -750,14 +712,34 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
                 u1 = __fsub_rn(xr[q + 1], Kc);
                 v0 = __fsub_rn(yr[q], Kc);
                 v1 = __fsub_rn(yr[q + 1], Kc);
-                if constexpr (DIAG) bv = __fadd_rn(bv, __fadd_rn(v0, v1));
             } else {
                 u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
                 u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
                 v0 = __fsub_rn(xr[q], yr[q]);
                 v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
             }
-            if constexpr (DIAG) bu = __fadd_rn(bu, __fadd_rn(u0, u1));
+            if constexpr (WT) {
+                const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q));
+                const float w1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q + 1));
+                if constexpr (DIAG && !XY) {
+                    const float a0 = __fmul_rn(w0, u0), a1 = __fmul_rn(w1, u1);
+                    const float b0 = __fmul_rn(w0, v0), b1 = __fmul_rn(w1, v1);
+                    bu = __fadd_rn(bu, __fadd_rn(a0, a1));
+                    if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(b0, b1));
+                    split_bf16x2(a0, a1, uwh[q / 2], uwl[q / 2]);
+                    split_bf16x2(b0, b1, vwh[q / 2], vwl[q / 2]);
+                } else if (wscale) {
+                    u0 = __fmul_rn(w0, u0);
+                    u1 = __fmul_rn(w1, u1);
+                    if constexpr (!XY) {
+                        v0 = __fmul_rn(w0, v0);
+                        v1 = __fmul_rn(w1, v1);
+                    }
+                }
+            } else if constexpr (DIAG) {
+                bu = __fadd_rn(bu, __fadd_rn(u0, u1));
+                if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(v0, v1));
+            }
             split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
             split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
         }
@@ -766,6 +748,12 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
             if constexpr (NRS) sum_v += (double)bv;
         }
         const int off = lcol * RPITCH + pfirst;
+        if constexpr (WT && DIAG && !XY) {  // PPT == 4: the weighted copies into slots 1 and 3
+            *reinterpret_cast<uint2 *>(&lds.op[buf][1][0][off]) = make_uint2(uwh[0], uwh[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][1][1][off]) = make_uint2(uwl[0], uwl[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][3][0][off]) = make_uint2(vwh[0], vwh[1]);
+            *reinterpret_cast<uint2 *>(&lds.op[buf][3][1][off]) = make_uint2(vwl[0], vwl[1]);
+        }
         if constexpr (PPT == 8) {
             *reinterpret_cast<uint4 *>(&lds.op[buf][su][0][off]) = make_uint4(uh[0], uh[1], uh[2], uh[3]);
             *reinterpret_cast<uint4 *>(&lds.op[buf][su][1][off]) = make_uint4(ul[0], ul[1], ul[2], ul[3]);
@@ -783,8 +771,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     // RSUB stages from zero (acc), which is then added to the span sum (sacc);
     // the span sum goes to the FP64 slot every RFLUSH pairs.  An element of a
     // span then carries about RSUB * 4 + RSPAN / RSUB sequential FP32
-    // roundings instead of RSPAN * 4.  It costs 64 more registers per lane,
-    // beyond the budget of two waves per SIMD (the compiler spills).  Without
+    // roundings instead of RSPAN * 4, for 64 more registers per lane.  Without
     // TL, acc is the span sum itself.
     floatx16 acc[2][2], sacc[2][2];
     auto zero_acc = [&]() __attribute__((always_inline)) {
@@ -881,14 +868,14 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     // is prepared into the other buffer from register set (s + 1) & 1, which is
     // then refilled with stage s + 3 ----
     const int n_even = n_stages > 0 ? (n_stages + 1) & ~1 : 2;  // at least one (zero) span: every slot is written
-    auto issue = [&](float *xr, float *yr, int t) __attribute__((always_inline)) {
+    auto issue = [&](float *xr, float *yr, float *wr, int t) __attribute__((always_inline)) {
         stage_rows(t);
-        issue_rows(xr, yr);
+        issue_rows(xr, yr, wr);
     };
-    issue(xa, ya, 0);
-    issue(xb, yb, 1);
-    prep(xa, ya, 0);
-    issue(xa, ya, 2);
+    issue(xa, ya, wa, 0);
+    issue(xb, yb, wb, 1);
+    prep(xa, ya, wa, 0);
+    issue(xa, ya, wa, 2);
     zero_acc();
     __syncthreads();
     bool first_flush = true;
@@ -897,12 +884,13 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     // rows of stage t are found first (scalar code with branches), so that the
     // matrix block is one basic block, the same for every stage: one copy of
     // the accumulators stays live across the loop.
-    auto half = [&](auto act_tag, int mb, float *xr, float *yr, int pb, int t) __attribute__((always_inline)) {
+    auto half = [&](auto act_tag, int mb, float *xr, float *yr, float *wr, int pb, int t)
+                    __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
         stage_rows(t);
         if constexpr (ACT) mfma(mb);
-        prep(xr, yr, pb);
-        issue_rows(xr, yr);
+        prep(xr, yr, wr, pb);
+        issue_rows(xr, yr, wr);
         if constexpr (ACT) interleave();
         __syncthreads();
     };
@@ -910,8 +898,8 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     auto run = [&](auto act_tag) __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
         for (int s = 0; s < n_even; s += 2) {
-            half(act_tag, 0, xb, yb, 1, s + 3);  // even stage s; prepare stage s + 1
-            half(act_tag, 1, xa, ya, 0, s + 4);  // odd stage s + 1; prepare stage s + 2
+            half(act_tag, 0, xb, yb, wb, 1, s + 3);  // even stage s; prepare stage s + 1
+            half(act_tag, 1, xa, ya, wa, 0, s + 4);  // odd stage s + 1; prepare stage s + 2
             const bool span_end = ((s + 2) % RSPAN) == 0 || s + 2 >= n_even;
             if constexpr (ACT && TL) {
                 if (span_end || ((s + 2) % RSUB) == 0) fold();
@@ -946,13 +934,13 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     }
 }
 
-// All tasks of the reversible bf16 moments (NRS: the symmetric tasks of the
-// non-reversible ones) in one launch.  The workgroups of one pair slice are
+// All I <= J tile tasks of one kind of bf16 moments (MODE: reversible,
+// non-reversible symmetric, lagged products) in one launch.  The workgroups of one pair slice are
 // consecutive in one XCD's dispatch order (off-diagonal tasks first, then the
 // diagonal ones), so the tasks that read the same frame rows run side by side
 // and share the XCD's L2: each frame comes from HBM about once per call
 // instead of once per kind of task.
-template <bool TL, int MODE>
+template <bool TL, int MODE, bool WT>
 __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     __shared__ __align__(16) RevLds lds;
     __shared__ double s_sum[2][4][RTB];
@@ -978,9 +966,9 @@ __global__ __launch_bounds__(RNT) void k_rev_bf16(Params p) {
     for (int i = 0; i < I; ++i) task += p.nb - i;
     task += J - I;
     if (I == J)
-        rev_body<true, TL, MODE>(p, lds, s_sum, I, J, task, slice);
+        rev_body<true, TL, MODE, WT>(p, lds, s_sum, I, J, task, slice);
     else
-        rev_body<false, TL, MODE>(p, lds, s_sum, I, J, task, slice);
+        rev_body<false, TL, MODE, WT>(p, lds, s_sum, I, J, task, slice);
 }
 
 // ------------------------------------------------------------------
@@ -993,7 +981,12 @@ __global__ __launch_bounds__(256) void k_wsum(Params p) {
     const int64_t np = p.pair_off[t + 1] - p.pair_off[t];
     const int64_t base = p.traj_off[t] + p.skip;
     double a = 0.0;
-    for (int64_t i = tid; i < np; i += 256) a += p.wts[base + i * p.stride];
+    const int64_t q0 = p.pair_off[t];
+    for (int64_t i = tid; i < np; i += 256) {
+        const double wv = p.wts[base + i * p.stride];
+        a += wv;
+        p.wpair[q0 + i] = (float)wv;
+    }
     s_part[tid] = a;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -1134,47 +1127,41 @@ template <int PREC, bool WT>
 static void launch_kind_w(const Params &p, hipStream_t s) {
     const int n_sym = p.n_sym;
     const unsigned g_sym = (unsigned)(cdiv((int64_t)n_sym * p.n_slices, 8) * 8);
-    if constexpr (PREC == EMMA_PREC_BF16X4 && !WT) {
+    if constexpr (PREC == EMMA_PREC_BF16X4) {
+        // the pipelined kernel: MODE 0 reversible, MODE 1 the symmetric
+        // non-reversible (or lag-0) matrices, MODE 2 the lagged products
         if (p.rev) {
             if (p.acc2)
-                k_rev_bf16<true, 0><<<g_sym, RNT, 0, s>>>(p);
+                k_rev_bf16<true, 0, WT><<<g_sym, RNT, 0, s>>>(p);
             else
-                k_rev_bf16<false, 0><<<g_sym, RNT, 0, s>>>(p);
+                k_rev_bf16<false, 0, WT><<<g_sym, RNT, 0, s>>>(p);
             return;
         }
-        if (p.lag == 0) {
-            // XX only: mode NRS with y = x (the YY waves only load and split)
+        if (p.acc2)
+            k_rev_bf16<true, 1, WT><<<g_sym, RNT, 0, s>>>(p);
+        else
+            k_rev_bf16<false, 1, WT><<<g_sym, RNT, 0, s>>>(p);
+        if (p.has_xy) {
             if (p.acc2)
-                k_rev_bf16<true, 1><<<g_sym, RNT, 0, s>>>(p);
+                k_rev_bf16<true, 2, WT><<<g_sym, RNT, 0, s>>>(p);
             else
-                k_rev_bf16<false, 1><<<g_sym, RNT, 0, s>>>(p);
-            return;
+                k_rev_bf16<false, 2, WT><<<g_sym, RNT, 0, s>>>(p);
         }
-        {
-            // C00 (and Ctt), then C0t: two launches of the pipelined kernel
-            // over the I <= J tiles
-            if (p.acc2) {
-                k_rev_bf16<true, 1><<<g_sym, RNT, 0, s>>>(p);
-                if (p.has_xy) k_rev_bf16<true, 2><<<g_sym, RNT, 0, s>>>(p);
-            } else {
-                k_rev_bf16<false, 1><<<g_sym, RNT, 0, s>>>(p);
-                if (p.has_xy) k_rev_bf16<false, 2><<<g_sym, RNT, 0, s>>>(p);
-            }
-            return;
+    } else {
+        // FP32 and FP64 tiles: the general kernel
+        if (p.lag == 0)
+            k_main<PREC, LAG0, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        else if (p.rev)
+            k_main<PREC, REV, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        else if (p.ctt)
+            k_main<PREC, NR_SYM, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        else
+            k_main<PREC, NR_SYM, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
+        if (p.has_xy) {
+            const int n_xy = p.nb * p.nb;
+            const unsigned g_xy = (unsigned)(cdiv((int64_t)n_xy * p.n_slices, 8) * 8);
+            k_main<PREC, NR_XY, 1, WT><<<g_xy, NTHR, 0, s>>>(p, n_sym, n_xy);
         }
-    }
-    if (p.lag == 0)
-        k_main<PREC, LAG0, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
-    else if (p.rev)
-        k_main<PREC, REV, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
-    else if (p.ctt)
-        k_main<PREC, NR_SYM, 2, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
-    else
-        k_main<PREC, NR_SYM, 1, WT><<<g_sym, NTHR, 0, s>>>(p, 0, n_sym);
-    if (p.has_xy) {
-        const int n_xy = p.nb * p.nb;
-        const unsigned g_xy = (unsigned)(cdiv((int64_t)n_xy * p.n_slices, 8) * 8);
-        k_main<PREC, NR_XY, 1, WT><<<g_xy, NTHR, 0, s>>>(p, n_sym, n_xy);
     }
 }
 
@@ -1224,7 +1211,7 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.has_xy = lag > 0 && !p.rev && (flags & EMMA_MOM_C0T) ? 1 : 0;
     p.n_tasks = p.n_sym + (p.has_xy ? p.nb * p.nb : 0);
     const bool weighted = (flags & EMMA_MOM_WEIGHTED) != 0;
-    if (precision == EMMA_PREC_BF16X4 && !weighted) {
+    if (precision == EMMA_PREC_BF16X4) {
         // one 8-wave workgroup per CU: 256 slices make the launch of
         // k_rev_bf16 a whole number of rounds (one per task) over the 256 CUs
         const int64_t cap = cdiv(n_rows > 0 ? n_rows : 1, 8192);
@@ -1246,6 +1233,7 @@ static void make_plan(Plan &pl, const float *X, int64_t n_rows, int d, const int
     p.wts = nullptr;
     p.wpart = weighted ? cv.take<double>((size_t)(ntraj > 0 ? ntraj : 1)) : nullptr;
     p.W = weighted ? cv.take<double>(1) : nullptr;
+    p.wpair = weighted ? cv.take<float>((size_t)(n_rows > 0 ? n_rows : 1)) : nullptr;  // P <= n_rows
     pl.ws_bytes = cv.bytes();
 }
 
@@ -1256,17 +1244,11 @@ using namespace emma;
 
 namespace emma {
 namespace mom {
-// EMMA_PREC_BF16X4_ACC: the reversible, unweighted moments run the bf16 kernel
-// with two-level accumulation; every other kind runs the FP64 tiles, which
-// meet the same accuracy.
-static int effective_precision(int precision, uint32_t flags, int lag, int *acc2) {
-    *acc2 = 0;
-    if (precision != EMMA_PREC_BF16X4_ACC) return precision;
-    if (!(flags & EMMA_MOM_WEIGHTED)) {
-        *acc2 = 1;
-        return EMMA_PREC_BF16X4;
-    }
-    return EMMA_PREC_FP64;
+// EMMA_PREC_BF16X4_ACC: the bf16 kernel with two-level accumulation (every
+// kind of moments runs the pipelined kernel in bf16).
+static int effective_precision(int precision, int *acc2) {
+    *acc2 = precision == EMMA_PREC_BF16X4_ACC ? 1 : 0;
+    return *acc2 ? EMMA_PREC_BF16X4 : precision;
 }
 }  // namespace mom
 }  // namespace emma
@@ -1274,11 +1256,7 @@ static int effective_precision(int precision, uint32_t flags, int lag, int *acc2
 extern "C" size_t emma_moments_workspace_size(int64_t n_rows, int32_t d, int32_t ntraj, uint32_t flags,
                                               int32_t precision) {
     if (d <= 0 || ntraj < 0) return 0;
-    if (precision == EMMA_PREC_BF16X4_ACC) {  // either form may run: size for both
-        const size_t a = emma_moments_workspace_size(n_rows, d, ntraj, flags, EMMA_PREC_BF16X4);
-        const size_t b = emma_moments_workspace_size(n_rows, d, ntraj, flags, EMMA_PREC_FP64);
-        return a > b ? a : b;
-    }
+    if (precision == EMMA_PREC_BF16X4_ACC) precision = EMMA_PREC_BF16X4;  // the same plan
     mom::Plan pl, pl0;
     // lag only matters through lag == 0; size for the larger task set
     mom::make_plan(pl, nullptr, n_rows, d, nullptr, ntraj, 1, 1, 0, flags, precision, nullptr);
@@ -1306,7 +1284,7 @@ extern "C" int emma_moments_lagged_weighted(const float *X, int64_t n_rows, int3
     EMMA_REQUIRE(!(flags & EMMA_MOM_C0T) || Mxy, "emma_moments_lagged: Mxy is NULL");
     EMMA_REQUIRE(!(flags & EMMA_MOM_CTT) || Myy, "emma_moments_lagged: Myy is NULL");
     int acc2 = 0;
-    precision = mom::effective_precision(precision, flags, lag, &acc2);
+    precision = mom::effective_precision(precision, &acc2);
     mom::Plan pl;
     mom::make_plan(pl, X, n_rows, d, traj_offsets, ntraj, lag, stride, skip, flags, precision, nullptr);
     EMMA_REQUIRE(workspace && workspace_bytes >= pl.ws_bytes,

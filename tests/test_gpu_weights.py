@@ -20,7 +20,7 @@ from pyemma_amd._solvers import sort_by_norm
 
 pytestmark = pytest.mark.gpu
 
-PRECS = [_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_FP32, _native.EMMA_PREC_FP64]
+PRECS = [_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_FP32, _native.EMMA_PREC_FP64, _native.EMMA_PREC_BF16X4_ACC]
 
 
 def _gpu(trajs, weights, lag, prec, **kw):
@@ -40,9 +40,10 @@ def _close(a, b, prec):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("reversible", [True, False])
-def test_weighted_moments_vs_oracle(prec, reversible):
-    rng = np.random.default_rng(11)
-    trajs = [(rng.standard_normal((T, 37)).cumsum(0) * 0.1 + 2.0).astype(np.float32) for T in (900, 41, 333)]
+@pytest.mark.parametrize("d", [37, 300])
+def test_weighted_moments_vs_oracle(prec, reversible, d):
+    rng = np.random.default_rng(11 + d)
+    trajs = [(rng.standard_normal((T, d)).cumsum(0) * 0.1 + 2.0).astype(np.float32) for T in (900, 41, 333)]
     weights = [rng.uniform(-0.2, 1.5, size=len(a)) for a in trajs]
     lag = 7
     ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=reversible, bessel=False,
@@ -50,8 +51,11 @@ def test_weighted_moments_vs_oracle(prec, reversible):
     w, sx, sy, Mxx, Mxy, _ = _gpu(trajs, weights, lag, prec, reversible=reversible, remove_mean=True, c0t=True)
     np.testing.assert_allclose(w[0], ref["w"], rtol=1e-12)
     _close(sx / w[0], ref["mean"], prec)
+    if not reversible:
+        _close(sy / w[0], ref["mean_tau"], prec)
     _close(Mxx / w[0], ref["C00"], prec)
     _close(Mxy / w[0], ref["C0t"], prec)
+    np.testing.assert_array_equal(Mxx, Mxx.T)
 
 
 def _transform_C0(C, epsilon):
