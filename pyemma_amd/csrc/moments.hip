@@ -685,10 +685,14 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         }
     };
     auto issue_rows = [&](float *xr, float *yr, float *wr) __attribute__((always_inline)) {
+        // the 32-bit column offset is made opaque here, so that each load is
+        // (scalar row pointer + 32-bit lane offset) and needs no 64-bit vector add
+        uint32_t co = coff;
+        asm volatile("" : "+v"(co));
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
-            xr[q] = *reinterpret_cast<const float *>(rx[q] + coff);
-            yr[q] = *reinterpret_cast<const float *>(ry[q] + coff);
+            xr[q] = *reinterpret_cast<const float *>(rx[q] + co);
+            yr[q] = *reinterpret_cast<const float *>(ry[q] + co);
         }
         if constexpr (WT) {
             // pairs past the slice (U = V = 0) read any valid weight
