@@ -58,6 +58,23 @@ def test_weighted_moments_vs_oracle(prec, reversible, d):
     np.testing.assert_array_equal(Mxx, Mxx.T)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("d", [37, 300])
+def test_weighted_lag0_vs_oracle(prec, d):
+    """Weighted instantaneous moments (moments_XX with weights, moments.py:601-698)."""
+    rng = np.random.default_rng(17 + d)
+    trajs = [(rng.standard_normal((T, d)) + 1.5).astype(np.float32) for T in (700, 90)]
+    weights = [rng.uniform(0.1, 2.0, size=len(a)) for a in trajs]
+    ref = om.covariance_lagged(trajs, 0, c00=True, c0t=False, remove_data_mean=True, bessel=False,
+                               weights=weights, stride=2, skip=1)
+    w, sx, _, Mxx, _, _ = _gpu(trajs, weights, 0, prec, stride=2, skip=1, reversible=False, remove_mean=True,
+                               c0t=False)
+    np.testing.assert_allclose(w[0], ref["w"], rtol=1e-12)
+    _close(sx / w[0], ref["mean"], prec)
+    _close(Mxx / w[0], ref["C00"], prec)
+    np.testing.assert_array_equal(Mxx, Mxx.T)
+
+
 def _transform_C0(C, epsilon):
     """test_tica.py:66-76."""
     d, V = scl.eigh(C)
