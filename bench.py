@@ -176,6 +176,7 @@ def main():
 
     kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
     del xyz
+    kern["tica_projected"] = projected_leg(args, X, off_dev, mom_ms)
 
     plain = None
     if not args.no_plain:
@@ -266,6 +267,50 @@ def adjacent_legs(args, X, off_dev, offsets, labels, xyz):
     log("featurize %.2f ms, counts %.2f ms, vamp moments %.2f ms"
         % (out["featurize"]["ms"], out["counts"]["ms"], out["vamp_moments"]["ms"]))
     torch.cuda.synchronize()
+    return out
+
+
+def projected_leg(args, X, off_dev, mom_ms, m=32, n_seed=200_000):
+    """The TICA-projected variant of the pipeline (tica, then k-means and
+    assignment on its output): every frame projected onto the m leading
+    kinetic-map TICA components (Y = (X - mean) R[:, :m] with R scaled by the
+    eigenvalues, FP64 math, float32 out), then assigned to k centers that
+    kmeans++ picks from a subsample of Y.  Reports the projection and
+    assignment times and the pipeline rate N / (moments + projection +
+    assignment) beside the headline."""
+    import torch
+
+    from pyemma_amd import _native, _solvers, autotune
+
+    N, d = X.shape
+    w, sx, _, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True,
+                                                    c0t=True, precision=_native.EMMA_PREC_BF16X4_ACC)
+    wh = float(w.item())
+    l, R = _solvers.eig_corr(Mxx.cpu().numpy() / wh, Mxy.cpu().numpy() / wh, 1e-6, sign_maxelement=True)
+    m = min(m, R.shape[1])
+    V = np.ascontiguousarray(R[:, :m] * l[None, :m])  # kinetic map
+    mean_dev = (sx / w).contiguous()
+    V_dev = torch.as_tensor(V, device=X.device)
+    Y = _native.project(X, mean_dev, V_dev, m)
+    ms_proj = _timed(lambda: _native.project(X, mean_dev, V_dev, m))
+    rng = np.random.default_rng(3)
+    sub = Y[torch.as_tensor(np.sort(rng.choice(N, size=min(n_seed, N), replace=False)), device=X.device)]
+    C, _ = _native.kmeanspp(sub.contiguous(), args.k, 7)
+    autotune.tune_assign(Y, C)
+    lab = torch.empty(N, dtype=torch.int32, device=X.device)
+    nre = torch.zeros(1, dtype=torch.int64, device=X.device)
+    ms_asg = _timed(lambda: _native.assign(Y, C, labels=lab, n_rechecked=nre))
+    torch.cuda.synchronize()
+    total = mom_ms + ms_proj + ms_asg
+    proj_bytes = 4 * (d + m) * N
+    out = {"dim": m, "k": args.k, "project_ms": ms_proj,
+           "project_gbs": proj_bytes / (ms_proj * 1e-3) / 1e9,
+           "assign_ms": ms_asg, "rechecked_frames": int(nre.item()),
+           "pipeline_ms": total, "pipeline_frames_per_s": N / (total * 1e-3)}
+    log("tica-projected (m=%d): project %.2f ms, assign %.2f ms, pipeline %.1f M frames/s"
+        % (m, ms_proj, ms_asg, out["pipeline_frames_per_s"] / 1e6))
+    del Y, sub, lab
+    torch.cuda.empty_cache()
     return out
 
 
