@@ -47,35 +47,35 @@ __global__ __launch_bounds__(256) void k_count_nonfinite(const float *__restrict
 __global__ void k_zero_i64(int64_t *p) { *p = 0; }
 
 // ------------------------------------------------------------------
-// TICA projection: Y[i, :m] = (X[i] - mean) V, FP64 accumulation.
-// One thread per frame (PROJ_ROWS = 256 frames per workgroup) holds MB
-// output accumulators; each output is a sequential FP64 fma chain over the
-// dims in order, x_c = (double)X[i, c] - mean[c].  The frames' dims are
-// staged in chunks of PROJ_DC: coalesced loads, written transposed into LDS
-// so that a thread reads its own frame's dims conflict-free; the chunk's V
-// rows and mean sit in LDS and are read as broadcasts.  The next chunk's
-// frames, V rows and mean are loaded while this chunk computes.
+// TICA projection: Y[i, :m] = (X[i] - mean) V on FP64 MFMA
+// (v_mfma_f64_16x16x4f64: D[16 frames][16 outputs] += A[16][4 dims] B[4][16]).
+// A workgroup holds 256 frames (4 waves x 4 row tiles of 16) and, per pass,
+// NT column tiles of 16 outputs.  The frames' dims are staged in chunks of
+// PROJ_DC: coalesced loads, written transposed into LDS, together with the
+// chunk's rows of V (zero past d and m) and of the mean; the next chunk's
+// loads are issued before this chunk's MFMAs.  Lane layout (l = lane):
+// A[i][k] = x[l % 16][l / 16] - mean, B[k][j] = V[l / 16][l % 16], D holds
+// rows 4 r + l / 16, r = 0..3, of column l % 16.  Every product and sum is
+// FP64; the outputs are rounded to float32 once.
 // ------------------------------------------------------------------
-constexpr int PROJ_ROWS = 256;  // frames per workgroup (one per thread)
-constexpr int PROJ_DC = 32;     // dims per staged chunk
-constexpr int PROJ_MB = 8;      // outputs per pass over the dims
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int PROJ_ROWS = 256;                // frames per workgroup
+constexpr int PROJ_DC = 32;                   // dims per staged chunk
+constexpr int PROJ_XS = PROJ_ROWS + 17;       // LDS row stride of the staged frames (floats)
+constexpr int PROJ_VS = 48;                   // LDS row stride of the staged V rows (doubles)
 
-// MB outputs per pass over the dims; the chunk's V rows are loaded with its
-// frames, one chunk ahead.  With MB = 8 a pass reads the frames at about
-// 3.6 TB/s (10^7 x 300 frames: 3.4 ms); wider blocks make the broadcast V
-// reads from LDS the limit (16 outputs: 7 ms per pass), so m > 8 takes
-// ceil(m / 8) passes.
-template <int MB>
+template <int NT>
 __global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__ X, int64_t n, int d,
                                                        const double *__restrict__ mean,
                                                        const double *__restrict__ V, int ldv, int m,
                                                        float *__restrict__ Y) {
-    __shared__ float s_x[PROJ_DC][PROJ_ROWS + 1];
-    __shared__ double s_v[PROJ_DC][MB];
+    __shared__ float s_x[PROJ_DC][PROJ_XS];
+    __shared__ double s_v[PROJ_DC][PROJ_VS];
     __shared__ double s_mean[PROJ_DC];
-    constexpr int PER = PROJ_DC;                           // frame loads per thread per chunk
-    constexpr int VPER = (PROJ_DC * MB + PROJ_ROWS - 1) / PROJ_ROWS;  // V loads per thread per chunk
-    const int tid = threadIdx.x;
+    constexpr int PER = PROJ_DC;                            // frame loads per thread per chunk
+    constexpr int NJ = 16 * NT;                             // outputs per pass
+    constexpr int VPER = PROJ_DC * NJ / PROJ_ROWS;          // V loads per thread per chunk
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * PROJ_ROWS;
     const int nr = (int)((n - r0) < PROJ_ROWS ? (n - r0) : PROJ_ROWS);
     const int n_chunks = (d + PROJ_DC - 1) / PROJ_DC;
@@ -83,53 +83,78 @@ __global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__
     const int lcol = tid & (PROJ_DC - 1);
     const int lrow0 = tid / PROJ_DC;
     const float *xb = X + r0 * (int64_t)d;
+    const uint32_t rstep = (uint32_t)(PROJ_ROWS / PROJ_DC) * (uint32_t)d;  // 8 rows
+    // last element of the workgroup's nr frames: a load of a row >= nr is
+    // clamped to it (every load stays inside the block; those rows belong to
+    // outputs that are not stored); dims >= d are staged as zero
+    const uint32_t off_max = (uint32_t)(nr * d - 1);
     float pre[PER];
     double prev[VPER], prem = 0.0;
-    auto load_chunk = [&](int ch, int j0, int mb) __attribute__((always_inline)) {
+    auto load_chunk = [&](int ch, int j0) __attribute__((always_inline)) {
         const int c0 = ch * PROJ_DC;
-        const int c = c0 + lcol;
+        const bool cin = c0 + lcol < d;
+        uint32_t off = (uint32_t)(lrow0 * d + c0 + lcol);  // < 256 d elements from xb
 #pragma unroll
         for (int it = 0; it < PER; ++it) {
-            const int r = lrow0 + it * (PROJ_ROWS / PROJ_DC);
-            pre[it] = (r < nr && c < d) ? xb[(uint32_t)(r * d + c)] : 0.0f;  // < 256 d elements
+            const float v = xb[off < off_max ? off : off_max];
+            pre[it] = cin ? v : 0.0f;
+            off += rstep;
+            asm volatile("" : "+v"(off));
         }
 #pragma unroll
         for (int u = 0; u < VPER; ++u) {
-            const int e = u * PROJ_ROWS + tid, cc = e / MB, jj = e - cc * MB;
-            prev[u] = (e < PROJ_DC * MB && c0 + cc < d && jj < mb) ? V[(int64_t)(c0 + cc) * ldv + j0 + jj] : 0.0;
+            const int e = u * PROJ_ROWS + tid, cc = e / NJ, jj = e - cc * NJ;
+            prev[u] = (c0 + cc < d && j0 + jj < m) ? V[(int64_t)(c0 + cc) * ldv + j0 + jj] : 0.0;
         }
         if (tid < PROJ_DC) prem = c0 + tid < d ? mean[c0 + tid] : 0.0;
     };
-    for (int j0 = 0; j0 < m; j0 += MB) {
-        const int mb = (m - j0) < MB ? (m - j0) : MB;
-        double acc[MB];
+    const int li = lane & 15, lk = lane >> 4;
+    for (int j0 = 0; j0 < m; j0 += NJ) {
+        v4d acc[4][NT];
 #pragma unroll
-        for (int j = 0; j < MB; ++j) acc[j] = 0.0;
-        load_chunk(0, j0, mb);
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
+        load_chunk(0, j0);
         for (int ch = 0; ch < n_chunks; ++ch) {
-            const int nc = (d - ch * PROJ_DC) < PROJ_DC ? (d - ch * PROJ_DC) : PROJ_DC;
             __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
             for (int it = 0; it < PER; ++it) s_x[lcol][lrow0 + it * (PROJ_ROWS / PROJ_DC)] = pre[it];
 #pragma unroll
             for (int u = 0; u < VPER; ++u) {
-                const int e = u * PROJ_ROWS + tid;
-                if (e < PROJ_DC * MB) s_v[e / MB][e % MB] = prev[u];
+                const int e = u * PROJ_ROWS + tid, cc = e / NJ, jj = e - cc * NJ;
+                s_v[cc][jj] = prev[u];
             }
             if (tid < PROJ_DC) s_mean[tid] = prem;
             __syncthreads();
-            if (ch + 1 < n_chunks) load_chunk(ch + 1, j0, mb);
+            if (ch + 1 < n_chunks) load_chunk(ch + 1, j0);
 #pragma unroll 2
-            for (int c = 0; c < nc; ++c) {
-                const double x = (double)s_x[c][tid] - s_mean[c];
+            for (int ks = 0; ks < PROJ_DC / 4; ++ks) {
+                const int c = 4 * ks + lk;
+                const double mu = s_mean[c];
+                double a[4], b[NT];
 #pragma unroll
-                for (int j = 0; j < MB; ++j) acc[j] = fma(x, s_v[c][j], acc[j]);
+                for (int rt = 0; rt < 4; ++rt) a[rt] = (double)s_x[c][wave * 64 + rt * 16 + li] - mu;
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct) b[ct] = s_v[c][ct * 16 + li];
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < NT; ++ct)
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
             }
         }
-        if (tid < nr) {
-            float *y = Y + (r0 + tid) * (int64_t)m + j0;
-            for (int j = 0; j < mb; ++j) y[j] = (float)acc[j];
-        }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                const int j = j0 + ct * 16 + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int f = wave * 64 + rt * 16 + 4 * r + lk;
+                    if (f < nr && j < m) Y[(r0 + f) * (int64_t)m + j] = (float)acc[rt][ct][r];
+                }
+            }
     }
 }
 
@@ -161,8 +186,14 @@ extern "C" int emma_project(const float *X, int64_t n, int32_t d, const double *
                  (long long)n, d, m, ldv);
     if (n == 0 || m == 0) return EMMA_OK;
     EMMA_REQUIRE(X && mean && V && Y, "emma_project: NULL pointer");
+    // 32-bit element offsets inside a workgroup's block of 256 frames
+    EMMA_REQUIRE(d <= (1 << 22), "emma_project: d=%d too large (max %d)", d, 1 << 22);
     hipStream_t s = (hipStream_t)stream;
-    k_project<PROJ_MB><<<(unsigned)cdiv(n, PROJ_ROWS), PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+    const unsigned grid = (unsigned)cdiv(n, PROJ_ROWS);
+    if (m <= 16)
+        k_project<1><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+    else
+        k_project<2><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
     return check_launch("emma_project");
 }
 

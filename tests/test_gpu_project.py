@@ -30,6 +30,15 @@ def test_projection_vs_fp64(n, d, m, ldv):
     assert np.all(np.abs(got.astype(np.float64) - ref) <= tol)
 
 
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 16, 17, 24, 25, 31, 40, 57, 64])
+def test_projection_output_blocks(m):
+    """Every output tiling of the kernel: one 16-output tile (m <= 16), two
+    (m <= 32), passes of 32 (m > 32), each with a partial last tile; 700
+    frames (a partial last workgroup) of 45 dims (a partial last dim chunk),
+    ldv = m + 3."""
+    test_projection_vs_fp64(700, 45, m, m + 3)
+
+
 def test_projection_empty():
     t = _native.torch()
     dev = _native.device()
