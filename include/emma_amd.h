@@ -202,7 +202,8 @@ int emma_lloyd(const float *X, int64_t n, int32_t d, float *C, int32_t k, int32_
 
 /*
  * Y = (X - mean) V[:, :m], computed in FP64 and rounded to float32.
- * mean (d,) FP64, V (d, ldv) FP64 row-major (columns 0..m-1 used), Y (n, m).
+ * mean (d,) FP64, V (d, ldv) FP64 row-major (columns 0..m-1 used), Y (n, m);
+ * d <= 2^22.
  *
  * Replaces TICABase._transform_array
  * (coordinates/transform/_tica_base.py:117-133).

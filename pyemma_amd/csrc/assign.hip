@@ -57,6 +57,7 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
     __shared__ float s_best[16][PF];
     __shared__ int s_bidx[16][PF];
     __shared__ int64_t s_row[PF];
+    __shared__ int s_nan0[PF];  // distance to center 0 is NaN
 
     const int tid = threadIdx.x;
     const int fg = tid >> 4;  // frames fg*4 .. +3
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
     if (tid < PF) {
         const int64_t f = f0 + tid;
         s_row[tid] = f < nrows ? (list ? list[f] : f) : -1;
+        s_nan0[tid] = 0;
     }
     __syncthreads();
 
@@ -111,6 +113,13 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
                     for (int b = 0; b < 4; ++b) acc[a][b] = sq_step(acc[a][b], xa[a], ca[b]);
             }
         }
+        // the reference loop starts from best = d(x, c_0): a NaN there is
+        // never replaced, so the label stays 0
+        if (c0 == 0 && cg == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                if (acc[a][0] != acc[a][0]) s_nan0[fg * 4 + a] = 1;
+        }
         // centers of this thread in increasing index order: strict '<'
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -143,8 +152,9 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
                 bi = vi;
             }
         }
-        // every distance +inf: the reference loop keeps center 0
-        if (s_row[tid] >= 0) labels[s_row[tid]] = bi == 0x7fffffff ? 0 : bi;
+        // every distance +inf, or the distance to center 0 NaN: the
+        // reference loop keeps center 0
+        if (s_row[tid] >= 0) labels[s_row[tid]] = (bi == 0x7fffffff || s_nan0[tid]) ? 0 : bi;
     }
     }
 }

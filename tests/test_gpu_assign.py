@@ -59,6 +59,37 @@ def test_exact_ties_first_index_wins(mode):
         assert nre >= 20  # duplicated centers always tie
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("bad_center", [None, 0, 3])
+def test_non_finite_rows_and_centers(mode, bad_center):
+    """NaN and inf in frames (and NaN in one center): the reference loop starts
+    from center 0 and replaces it only on a smaller distance, so a NaN
+    distance never wins and a NaN distance to center 0 keeps label 0."""
+    X, C = _data(700, 20, 40, seed=11)
+    X[5, 3] = np.nan
+    X[17, :] = np.inf
+    X[18, 0] = -np.inf
+    X[40:44, 7] = 3e19  # squares overflow to inf for every center
+    if bad_center is not None:
+        C[bad_center, 2] = np.nan
+    ref = on.assign(X, C)
+    np.testing.assert_array_equal(_run(X, C, mode)[0], ref)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_listed_candidate_after_a_tie(mode):
+    """Exact ties where the later center of a tied pair is also the nearer in
+    the MFMA pass's rounding: the first minimum in index order must win."""
+    rng = np.random.default_rng(2)
+    base = rng.standard_normal((16, 64)).astype(np.float32)
+    C = np.concatenate([base, base], axis=0)  # center j and j + 16 identical
+    X = (base[rng.integers(0, 16, 3000)] + rng.standard_normal((3000, 64)).astype(np.float32) * 0.05).astype(np.float32)
+    ref = on.assign(X, C)
+    lab, nre = _run(X, C, mode)
+    np.testing.assert_array_equal(lab, ref)
+    assert np.all(lab < 16)
+
+
 def test_golden_hmm_labels(golden_hmm):
     g = golden_hmm
     for mode in MODES:
