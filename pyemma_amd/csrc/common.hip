@@ -62,17 +62,22 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PROJ_ROWS = 256;                // frames per workgroup
 constexpr int PROJ_DC = 32;                   // dims per staged chunk
 constexpr int PROJ_XS = PROJ_ROWS + 17;       // LDS row stride of the staged frames (floats)
-constexpr int PROJ_VS = 48;                   // LDS row stride of the staged V rows (doubles)
-
+// LDS row stride of the staged V rows (doubles): NT tiles of 16 plus 2
+// (rows of one read start 2 banks apart)
 template <int NT>
+constexpr int proj_vs() { return 16 * NT + 2; }
+
+// VEC: frames loaded as 16-byte vectors (d % 4 == 0, X 16-byte aligned):
+// thread t loads dims 4 (t % 8) .. + 3 of the chunk for rows t / 8 + 32 it
+template <int NT, bool VEC>
 __global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__ X, int64_t n, int d,
                                                        const double *__restrict__ mean,
                                                        const double *__restrict__ V, int ldv, int m,
                                                        float *__restrict__ Y) {
     __shared__ float s_x[PROJ_DC][PROJ_XS];
-    __shared__ double s_v[PROJ_DC][PROJ_VS];
+    __shared__ double s_v[PROJ_DC][proj_vs<NT>()];
     __shared__ double s_mean[PROJ_DC];
-    constexpr int PER = PROJ_DC;                            // frame loads per thread per chunk
+    constexpr int PER = PROJ_DC;                            // frame values per thread per chunk
     constexpr int NJ = 16 * NT;                             // outputs per pass
     constexpr int VPER = PROJ_DC * NJ / PROJ_ROWS;          // V loads per thread per chunk
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -90,16 +95,35 @@ __global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__
     const uint32_t off_max = (uint32_t)(nr * d - 1);
     float pre[PER];
     double prev[VPER], prem = 0.0;
+    // vector staging map: float4 q = t % 8 of rows t / 8 + 32 it
+    const int vq = tid & 7, vrow0 = tid >> 3;
+    const uint32_t vstep = 32u * (uint32_t)d;
     auto load_chunk = [&](int ch, int j0) __attribute__((always_inline)) {
         const int c0 = ch * PROJ_DC;
-        const bool cin = c0 + lcol < d;
-        uint32_t off = (uint32_t)(lrow0 * d + c0 + lcol);  // < 256 d elements from xb
+        if constexpr (VEC) {
+            const bool cin = c0 + 4 * vq < d;  // d % 4 == 0: a vector is all in or all out
+            const uint32_t vmax = off_max - 3;  // last vector of the block
+            uint32_t off = (uint32_t)(vrow0 * d + c0 + 4 * vq);
 #pragma unroll
-        for (int it = 0; it < PER; ++it) {
-            const float v = xb[off < off_max ? off : off_max];
-            pre[it] = cin ? v : 0.0f;
-            off += rstep;
-            asm volatile("" : "+v"(off));
+            for (int it = 0; it < PER / 4; ++it) {
+                const float4 v = *reinterpret_cast<const float4 *>(xb + (off < vmax ? off : vmax));
+                pre[4 * it + 0] = cin ? v.x : 0.0f;
+                pre[4 * it + 1] = cin ? v.y : 0.0f;
+                pre[4 * it + 2] = cin ? v.z : 0.0f;
+                pre[4 * it + 3] = cin ? v.w : 0.0f;
+                off += vstep;
+                asm volatile("" : "+v"(off));
+            }
+        } else {
+            const bool cin = c0 + lcol < d;
+            uint32_t off = (uint32_t)(lrow0 * d + c0 + lcol);  // < 256 d elements from xb
+#pragma unroll
+            for (int it = 0; it < PER; ++it) {
+                const float v = xb[off < off_max ? off : off_max];
+                pre[it] = cin ? v : 0.0f;
+                off += rstep;
+                asm volatile("" : "+v"(off));
+            }
         }
 #pragma unroll
         for (int u = 0; u < VPER; ++u) {
@@ -118,8 +142,15 @@ __global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__
         load_chunk(0, j0);
         for (int ch = 0; ch < n_chunks; ++ch) {
             __syncthreads();  // the previous chunk's LDS reads are done
+            if constexpr (VEC) {
 #pragma unroll
-            for (int it = 0; it < PER; ++it) s_x[lcol][lrow0 + it * (PROJ_ROWS / PROJ_DC)] = pre[it];
+                for (int it = 0; it < PER / 4; ++it)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) s_x[4 * vq + e][vrow0 + 32 * it] = pre[4 * it + e];
+            } else {
+#pragma unroll
+                for (int it = 0; it < PER; ++it) s_x[lcol][lrow0 + it * (PROJ_ROWS / PROJ_DC)] = pre[it];
+            }
 #pragma unroll
             for (int u = 0; u < VPER; ++u) {
                 const int e = u * PROJ_ROWS + tid, cc = e / NJ, jj = e - cc * NJ;
@@ -190,10 +221,18 @@ extern "C" int emma_project(const float *X, int64_t n, int32_t d, const double *
     EMMA_REQUIRE(d <= (1 << 22), "emma_project: d=%d too large (max %d)", d, 1 << 22);
     hipStream_t s = (hipStream_t)stream;
     const unsigned grid = (unsigned)cdiv(n, PROJ_ROWS);
-    if (m <= 16)
-        k_project<1><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
-    else
-        k_project<2><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+    const bool vec = d % 4 == 0 && ((uintptr_t)X & 15) == 0;
+    if (m <= 16) {
+        if (vec)
+            k_project<1, true><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+        else
+            k_project<1, false><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+    } else {
+        if (vec)
+            k_project<2, true><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+        else
+            k_project<2, false><<<grid, PROJ_ROWS, 0, s>>>(X, n, d, mean, V, ldv, m, Y);
+    }
     return check_launch("emma_project");
 }
 
