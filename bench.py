@@ -177,6 +177,7 @@ def main():
     kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
     del xyz
     kern["tica_projected"] = projected_leg(args, X, off_dev, mom_ms)
+    kern["kmeans"] = kmeans_leg(args, X, sub, cpu=not args.no_cpu)
 
     plain = None
     if not args.no_plain:
@@ -311,6 +312,65 @@ def projected_leg(args, X, off_dev, mom_ms, m=32, n_seed=200_000):
         % (m, ms_proj, ms_asg, out["pipeline_frames_per_s"] / 1e6))
     del Y, sub, lab
     torch.cuda.empty_cache()
+    return out
+
+
+def kmeans_leg(args, X, sub, cpu=True, seed=12345, n_lloyd=1_000_000, iters=5, n_cpu=20_000):
+    """kmeans++ seeding and Lloyd iterations on the bench data.  kmeans++:
+    k centers from the subsample (each round after the first reads the n
+    frames once and the D^2 state: B = (4 d + 8) n per round).  Lloyd:
+    `iters` iterations with tol = 0 on the first n_lloyd frames from the
+    kmeans++ centers (each iteration an assignment, 2 d k n flops, then the
+    FP64 cost and the center update).  With cpu: the oracle on n_cpu frames
+    of the subsample (kmeans++ with the same k and seed, one Lloyd
+    iteration), scaled linearly in the frames, and whether the GPU's
+    kmeans++ picks on those frames equal the oracle's."""
+    import torch
+
+    from pyemma_amd import _native
+
+    n, d = sub.shape
+    k = args.k
+    ms_pp = _timed(lambda: _native.kmeanspp(sub, k, seed), reps=1)
+    C0, _ = _native.kmeanspp(sub, k, seed)
+    Xl = X[:n_lloyd]
+    res = {}
+
+    def run_lloyd():
+        res["out"] = _native.lloyd(Xl, C0.clone(), iters, 0.0)
+
+    ms_ll = _timed(run_lloyd, reps=1)
+    n_iter = int(res["out"][2].item())
+    pp_bytes = (4 * d + 8) * n * (k - 1)
+    f_ll = 2.0 * d * k * Xl.shape[0] * n_iter
+    out = {"kmeanspp": {"frames": n, "dim": d, "k": k, "ms": ms_pp, "bytes": pp_bytes,
+                        "gbs": pp_bytes / (ms_pp * 1e-3) / 1e9, "hbm_frac": pp_bytes / (ms_pp * 1e-3) / 8e12},
+           "lloyd": {"frames": int(Xl.shape[0]), "dim": d, "k": k, "iterations": n_iter, "ms": ms_ll,
+                     "ms_per_iteration": ms_ll / max(n_iter, 1), "assign_tflops": f_ll / (ms_ll * 1e-3) / 1e12}}
+    log("kmeans++ k=%d on %d frames %.1f ms (%.0f GB/s); lloyd %d iterations on %d frames %.1f ms"
+        % (k, n, ms_pp, out["kmeanspp"]["gbs"], n_iter, Xl.shape[0], ms_ll))
+    if cpu:
+        from oracle import native as on
+
+        cores = len(os.sched_getaffinity(0))
+        Xs_dev = sub[:n_cpu].contiguous()
+        Xs = Xs_dev.cpu().numpy()
+        t0 = time.perf_counter()
+        Cc, idx_c = on.kmeanspp(Xs, k, seed, nthreads=cores)
+        t_pp = time.perf_counter() - t0
+        _, idx_g = _native.kmeanspp(Xs_dev, k, seed)
+        same = bool(np.array_equal(idx_g.cpu().numpy(), idx_c))
+        t0 = time.perf_counter()
+        on.lloyd(Xs, Cc, 1, 0.0, nthreads=cores)
+        t_ll = time.perf_counter() - t0
+        out["cpu"] = {"kind": "port", "cores": cores,
+                      "sample": "kmeans++ k=%d on %d frames (%.2f s); one Lloyd iteration on %d frames (%.2f s); "
+                                "scaled linearly in the frames" % (k, n_cpu, t_pp, n_cpu, t_ll),
+                      "kmeanspp_ms": t_pp * 1e3 * n / n_cpu,
+                      "lloyd_ms": t_ll * 1e3 * Xl.shape[0] / n_cpu * max(n_iter, 1),
+                      "kmeanspp_picks_identical": same}
+        log("kmeans cpu (oracle, %d cores): kmeans++ %.2f s, lloyd iteration %.2f s on %d frames; picks identical %s"
+            % (cores, t_pp, t_ll, n_cpu, same))
     return out
 
 
