@@ -126,7 +126,10 @@ size_t emma_assign_workspace_size(int64_t n, int32_t d, int32_t k, int32_t mode)
 
 /*
  * labels[i] = argmin_j sum_c (X[i,c] - C[j,c])^2, the sum taken in FP32 in
- * index order without fused multiply-add, first minimum wins.
+ * index order without fused multiply-add, first minimum wins.  As in the
+ * reference loop (best = distance to center 0, replaced only by a smaller
+ * one): a NaN distance never wins, a NaN distance to center 0 keeps label 0,
+ * and a row whose distances are all +inf gets label 0.
  *
  * Replaces ClusterModel(cluster_centers, metric='euclidean').transform(X)
  * as called from AbstractClustering._transform_array
