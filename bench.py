@@ -325,8 +325,6 @@ def kmeans_leg(args, X, sub, cpu=True, seed=12345, n_lloyd=1_000_000, iters=5, n
     of the subsample (kmeans++ with the same k and seed, one Lloyd
     iteration), scaled linearly in the frames, and whether the GPU's
     kmeans++ picks on those frames equal the oracle's."""
-    import torch
-
     from pyemma_amd import _native
 
     n, d = sub.shape
