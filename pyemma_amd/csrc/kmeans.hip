@@ -32,6 +32,7 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
 namespace km {
 
 constexpr int BLOCK = 4096;
+static_assert(BLOCK % 64 == 0, "a wave's frames lie in one block");
 constexpr int SUPER = 64;
 constexpr int SEG = 64;
 constexpr int CHUNK = 256;
@@ -325,7 +326,21 @@ __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X,
         di = dist_seq(x, c, d);
     }
     dist[i] = di;
-    atomicAdd(&blk_cnt[(i / BLOCK) * (int64_t)k + lab], 1);  // zeroed by k_fill_i32 beforehand
+    // histogram of labels per block (zeroed by k_fill_i32 beforehand), one
+    // atomic per run of equal labels in the wave: consecutive frames of a
+    // trajectory mostly share their label, and 64 lanes adding 1 to one
+    // counter serialize.  A wave's frames lie in one block (BLOCK % 64 == 0);
+    // the active lanes are 0 .. m - 1 (only a final partial wave has m < 64).
+    const int lane = threadIdx.x & 63;
+    const unsigned long long active = __ballot(1);
+    const int prev = __shfl_up(lab, 1, 64);
+    const bool start = lane == 0 || prev != lab;
+    const unsigned long long starts = __ballot(start);
+    if (start) {
+        const unsigned long long later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
+        const int end = later ? __builtin_ctzll(later) : __popcll(active);
+        atomicAdd(&blk_cnt[(i / BLOCK) * (int64_t)k + lab], end - lane);
+    }
 }
 
 // Per label: turn the blk_cnt column into exclusive prefixes over blocks and
