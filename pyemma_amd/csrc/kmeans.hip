@@ -433,8 +433,19 @@ __global__ __launch_bounds__(256) void k_lloyd_chunks(const float *__restrict__ 
     for (int i0 = 0; i0 < d; i0 += 64) {
         const int i = i0 + lane;
         double a = 0.0;
-        if (i < d)
-            for (int64_t m = m0; m < m1; ++m) a = a + (double)X[order[m] * d + i];
+        if (i < d) {
+            // members in order; the loads of 8 members are issued before
+            // their adds (same sum, one memory round trip per 8)
+            int64_t m = m0;
+            for (; m + 8 <= m1; m += 8) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = X[order[m + u] * d + i];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a = a + (double)v[u];
+            }
+            for (; m < m1; ++m) a = a + (double)X[order[m] * d + i];
+        }
         if (i < d) chunk_sums[ch * d + i] = a;
     }
 }
