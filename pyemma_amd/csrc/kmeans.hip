@@ -309,9 +309,7 @@ __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X,
     float di;
     if constexpr (VEC) {
         float acc = 0.0f;
-        for (int q = 0; q < d; q += 4) {
-            const float4 xv = *reinterpret_cast<const float4 *>(x + q);
-            const float4 cv = *reinterpret_cast<const float4 *>(c + q);
+        auto step4 = [&](const float4 xv, const float4 cv) __attribute__((always_inline)) {
             float t = __fsub_rn(xv.x, cv.x);
             acc = __fadd_rn(acc, __fmul_rn(t, t));
             t = __fsub_rn(xv.y, cv.y);
@@ -320,7 +318,21 @@ __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X,
             acc = __fadd_rn(acc, __fmul_rn(t, t));
             t = __fsub_rn(xv.w, cv.w);
             acc = __fadd_rn(acc, __fmul_rn(t, t));
+        };
+        // dims in order; the loads of 16 dims are issued before their terms
+        int q = 0;
+        for (; q + 16 <= d; q += 16) {
+            float4 xv[4], cv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xv[u] = *reinterpret_cast<const float4 *>(x + q + 4 * u);
+                cv[u] = *reinterpret_cast<const float4 *>(c + q + 4 * u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) step4(xv[u], cv[u]);
         }
+        for (; q < d; q += 4)
+            step4(*reinterpret_cast<const float4 *>(x + q), *reinterpret_cast<const float4 *>(c + q));
         di = acc;
     } else {
         di = dist_seq(x, c, d);
