@@ -80,12 +80,11 @@ __global__ __launch_bounds__(256) void k_kmpp_d2(const float *__restrict__ X, in
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float *x = X + i * d;
+    float v = first ? 0.0f : d2[i];
     float di;
     if constexpr (VEC) {
         float acc = 0.0f;
-        for (int k = 0; k < d; k += 4) {
-            const float4 xv = *reinterpret_cast<const float4 *>(x + k);
-            const float4 cv = *reinterpret_cast<const float4 *>(s_c + k);
+        auto step4 = [&](const float4 xv, const float4 cv) __attribute__((always_inline)) {
             float t = __fsub_rn(xv.x, cv.x);
             acc = __fadd_rn(acc, __fmul_rn(t, t));
             t = __fsub_rn(xv.y, cv.y);
@@ -94,12 +93,22 @@ __global__ __launch_bounds__(256) void k_kmpp_d2(const float *__restrict__ X, in
             acc = __fadd_rn(acc, __fmul_rn(t, t));
             t = __fsub_rn(xv.w, cv.w);
             acc = __fadd_rn(acc, __fmul_rn(t, t));
+        };
+        // dims in order; the frame loads of 16 dims are issued before their terms
+        int q = 0;
+        for (; q + 16 <= d; q += 16) {
+            float4 xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = *reinterpret_cast<const float4 *>(x + q + 4 * u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) step4(xv[u], *reinterpret_cast<const float4 *>(s_c + q + 4 * u));
         }
+        for (; q < d; q += 4)
+            step4(*reinterpret_cast<const float4 *>(x + q), *reinterpret_cast<const float4 *>(s_c + q));
         di = acc;
     } else {
         di = dist_seq(x, s_c, d);
     }
-    float v = d2[i];
     if (first || di < v) v = di;
     d2[i] = v;
 }
