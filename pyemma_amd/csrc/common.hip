@@ -70,10 +70,12 @@ constexpr int proj_vs() { return 16 * NT + 2; }
 // VEC: frames loaded as 16-byte vectors (d % 4 == 0, X 16-byte aligned):
 // thread t loads dims 4 (t % 8) .. + 3 of the chunk for rows t / 8 + 32 it
 template <int NT, bool VEC>
-__global__ __launch_bounds__(PROJ_ROWS) void k_project(const float *__restrict__ X, int64_t n, int d,
-                                                       const double *__restrict__ mean,
-                                                       const double *__restrict__ V, int ldv, int m,
-                                                       float *__restrict__ Y) {
+// one output tile: at least 3 workgroups per CU (the register budget of 3
+// waves per SIMD holds without spills); two tiles: the compiler's choice
+__global__ __launch_bounds__(PROJ_ROWS, NT == 1 ? 3 : 1) void k_project(const float *__restrict__ X, int64_t n,
+                                                                       int d, const double *__restrict__ mean,
+                                                                       const double *__restrict__ V, int ldv, int m,
+                                                                       float *__restrict__ Y) {
     __shared__ float s_x[PROJ_DC][PROJ_XS];
     __shared__ double s_v[PROJ_DC][proj_vs<NT>()];
     __shared__ double s_mean[PROJ_DC];
