@@ -184,10 +184,11 @@ size_t emma_lloyd_workspace_size(int64_t n, int32_t d, int32_t k, int32_t assign
 
 /*
  * Lloyd iterations on centers C (k,d) float32, updated in place.  Each
- * iteration: labels = emma_assign(X, C); cost = blocked FP64 sum of every
- * frame's FP32 distance to its center; C_j = mean of its members (FP64,
- * documented order; empty centers unchanged); stop when
- * |cost - prev| / cost <= tol (prev starts at 0).
+ * iteration: labels = emma_assign(X, C); C_j = mean of its members (FP64,
+ * documented order; empty centers unchanged); cost = blocked FP64 sum of
+ * every frame's FP32 distance to the updated center of its label (the
+ * k-means cost of the new clusters against their means, kmeans.py:67-75);
+ * stop when |cost - prev| / cost <= tol (prev starts at 0).
  * Device outputs: labels (n,) of the last assignment, inertias[max_iter]
  * (cost per iteration run), n_iter[1] (int32), converged[1] (int32).
  *

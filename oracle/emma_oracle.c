@@ -286,9 +286,11 @@ int oracle_update_centers(const float *X, int64_t n, int d, const int32_t *label
     return 0;
 }
 
-/* Lloyd iterations.  Iteration it: labels = assign(X, C); cost = blocked
- * FP64 sum (oracle_blocked_sum) of each frame's FP32 distance to its
- * assigned center; C = member means (oracle_update_centers);
+/* Lloyd iterations.  Iteration it: labels = assign(X, C); C = member means
+ * (oracle_update_centers); cost = blocked FP64 sum (oracle_blocked_sum) of
+ * each frame's FP32 distance to its center after the update, i.e. the
+ * k-means cost C(S) = sum_i sum_{x in S_i} ||x - mu_i||^2 of the clusters
+ * S_i just formed with mu_i their means (kmeans.py:67-75);
  * rel = |cost - prev| / cost (0 when cost == 0), prev starting at 0;
  * stop with converged = 1 when rel <= tol.  inertias[it] = cost.
  * Returns the number of iterations run, or -1 on failure. */
@@ -296,13 +298,17 @@ int oracle_lloyd(const float *X, int64_t n, int d, float *C, int k, int max_iter
                  int32_t *labels, double *inertias, int *converged, int nthreads) {
     float *dist = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
     if (!dist) return -1;
+    if (nthreads < 1) nthreads = 1;
     double prev = 0.0;
     int it = 0;
     *converged = 0;
     for (; it < max_iter; ++it) {
-        oracle_assign(X, n, d, C, k, labels, dist, nthreads);
-        double cost = oracle_blocked_sum(dist, n);
+        oracle_assign(X, n, d, C, k, labels, NULL, nthreads);
         if (oracle_update_centers(X, n, d, labels, C, k, NULL) != 0) { free(dist); return -1; }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            dist[i] = emma_dist_seq(X + i * (int64_t)d, C + (int64_t)labels[i] * d, d);
+        double cost = oracle_blocked_sum(dist, n);
         inertias[it] = cost;
         double rel = cost != 0.0 ? fabs(cost - prev) / cost : 0.0;
         prev = cost;

@@ -300,11 +300,13 @@ __global__ void k_lloyd_init(int32_t *done, double *prev_cost, int32_t *n_iter, 
     *converged = 0;
 }
 
-// One thread per frame: exact distance to the assigned center (the
-// sequential FP32 loop; VEC as in k_kmpp_d2, with C 16-byte aligned too) and
-// the frame's count in its block's per-label counts.  The block sums of the
-// distances (the oracle's cost order) follow in k_block_sums.
-template <bool VEC>
+// One thread per frame.  HIST: the frame's count in its block's per-label
+// counts (before the center update).  Otherwise: the exact distance to the
+// center of its label (the sequential FP32 loop; VEC as in k_kmpp_d2, with C
+// 16-byte aligned too), run after the update so that the cost is that of the
+// new clusters against their means.  The block sums of the distances (the
+// oracle's cost order) follow in k_block_sums.
+template <bool VEC, bool HIST>
 __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X, int64_t n, int d,
                                                     const float *__restrict__ C, int k,
                                                     const int32_t *__restrict__ labels, float *__restrict__ dist,
@@ -313,6 +315,24 @@ __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X,
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int lab = labels[i];
+    if constexpr (HIST) {
+        // histogram of labels per block (zeroed by k_fill_i32 beforehand), one
+        // atomic per run of equal labels in the wave: consecutive frames of a
+        // trajectory mostly share their label, and 64 lanes adding 1 to one
+        // counter serialize.  A wave's frames lie in one block (BLOCK % 64 == 0);
+        // the active lanes are 0 .. m - 1 (only a final partial wave has m < 64).
+        const int lane = threadIdx.x & 63;
+        const unsigned long long active = __ballot(1);
+        const int prev = __shfl_up(lab, 1, 64);
+        const bool start = lane == 0 || prev != lab;
+        const unsigned long long starts = __ballot(start);
+        if (start) {
+            const unsigned long long later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
+            const int end = later ? __builtin_ctzll(later) : __popcll(active);
+            atomicAdd(&blk_cnt[(i / BLOCK) * (int64_t)k + lab], end - lane);
+        }
+        return;
+    }
     const float *x = X + i * d;
     const float *c = C + (int64_t)lab * d;
     float di;
@@ -347,21 +367,6 @@ __global__ __launch_bounds__(256) void k_lloyd_dist(const float *__restrict__ X,
         di = dist_seq(x, c, d);
     }
     dist[i] = di;
-    // histogram of labels per block (zeroed by k_fill_i32 beforehand), one
-    // atomic per run of equal labels in the wave: consecutive frames of a
-    // trajectory mostly share their label, and 64 lanes adding 1 to one
-    // counter serialize.  A wave's frames lie in one block (BLOCK % 64 == 0);
-    // the active lanes are 0 .. m - 1 (only a final partial wave has m < 64).
-    const int lane = threadIdx.x & 63;
-    const unsigned long long active = __ballot(1);
-    const int prev = __shfl_up(lab, 1, 64);
-    const bool start = lane == 0 || prev != lab;
-    const unsigned long long starts = __ballot(start);
-    if (start) {
-        const unsigned long long later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
-        const int end = later ? __builtin_ctzll(later) : __popcll(active);
-        atomicAdd(&blk_cnt[(i / BLOCK) * (int64_t)k + lab], end - lane);
-    }
 }
 
 // Per label: turn the blk_cnt column into exclusive prefixes over blocks and
@@ -573,22 +578,24 @@ extern "C" int emma_lloyd(const float *X, int64_t n, int32_t d, float *C, int32_
     for (int it = 0; it < max_iter; ++it) {
         EMMA_TRY(assign_run(X, n, d, C, k, labels, assign_mode, nullptr, w.assign_ws, w.done, s));
         km::k_fill_i32<<<1024, 256, 0, s>>>(w.blk_cnt, nb * (int64_t)k, w.done);
-        if (vec)
-            km::k_lloyd_dist<true><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist, w.blk_cnt,
-                                                                        w.done);
-        else
-            km::k_lloyd_dist<false><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist, w.blk_cnt,
-                                                                         w.done);
-        km::k_block_sums<<<(unsigned)cdiv(nb, 4), 256, 0, s>>>(n, w.dist, w.bcost, w.done);
+        km::k_lloyd_dist<false, true><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist,
+                                                                            w.blk_cnt, w.done);
         km::k_lloyd_scan<<<1, 1024, 0, s>>>(k, nb, w.cnt, w.off, w.coff, w.blk_cnt, w.done);
         km::k_lloyd_scatter<<<(unsigned)nb, 64, sizeof(int32_t) * k, s>>>(n, k, labels, w.off, w.blk_cnt, w.order,
                                                                           w.done);
         const int64_t max_chunks = n / km::CHUNK + k + 1;
         km::k_lloyd_chunks<<<(unsigned)cdiv(max_chunks, 4), 256, 0, s>>>(X, d, k, w.off, w.coff, w.order,
                                                                          w.chunk_sums, w.done);
-        // the cost was taken with this iteration's centers; now move them, then
-        // record the cost (which may raise the convergence flag)
+        // move the centers to the member means, then take the cost of the new
+        // clusters against them and record it (which may raise the convergence flag)
         km::k_lloyd_update<<<(unsigned)cdiv(k, 4), 256, 0, s>>>(d, k, w.cnt, w.coff, w.chunk_sums, C, w.done);
+        if (vec)
+            km::k_lloyd_dist<true, false><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist,
+                                                                                w.blk_cnt, w.done);
+        else
+            km::k_lloyd_dist<false, false><<<(unsigned)cdiv(n, 256), 256, 0, s>>>(X, n, d, C, k, labels, w.dist,
+                                                                                 w.blk_cnt, w.done);
+        km::k_block_sums<<<(unsigned)cdiv(nb, 4), 256, 0, s>>>(n, w.dist, w.bcost, w.done);
         km::k_lloyd_cost_final<<<1, 1, 0, s>>>(nb, w.bcost, tol, it, w.prev_cost, inertias, n_iter, converged,
                                                w.done);
         EMMA_TRY(check_launch("lloyd iteration"));

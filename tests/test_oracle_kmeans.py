@@ -238,12 +238,38 @@ def test_lloyd_four_blocks():
     assert conv
 
 
+def _dist_to_own_center(X, C, lab):
+    """Sequential FP32 distance of each frame to the center of its label (NumPy, every op rounded)."""
+    X = X.astype(np.float32)
+    Cl = C.astype(np.float32)[lab]
+    acc = np.zeros(X.shape[0], dtype=np.float32)
+    for i in range(X.shape[1]):
+        t = (X[:, i] - Cl[:, i]).astype(np.float32)
+        acc = (acc + (t * t).astype(np.float32)).astype(np.float32)
+    return acc
+
+
 def test_lloyd_cost_nonincreasing_and_stopping():
+    """The cost of iteration t is the k-means objective of kmeans.py:67-75,
+    C(S) = sum_i sum_{x in S_i} ||x - mu_i||^2 with S_i the clusters formed by
+    that iteration's assignment and mu_i their means (the updated centers)."""
     rng = np.random.default_rng(12)
     X = rng.standard_normal((4000, 6)).astype(np.float32)
     C0, _ = on.kmeanspp(X, 20, 3)
     C, lab, inert, conv = on.lloyd(X, C0, 30, 0.0)
     assert np.all(np.diff(inert) <= 1e-9 * inert[:-1])
-    # cost of the last assignment equals the blocked sum of exact distances
-    _, dist = on.assign(X, on.lloyd(X, C0, len(inert) - 1, 0.0)[0], return_dists=True)
-    assert math.isclose(inert[-1], on.blocked_sum(dist), rel_tol=0, abs_tol=0)
+    # the last cost: the last labels against the centers they produced
+    assert math.isclose(inert[-1], on.blocked_sum(_dist_to_own_center(X, C, lab)), rel_tol=0, abs_tol=0)
+    # iteration by iteration: labels of the centers before the update, cost against the
+    # member means (the means minimize the cost of fixed clusters: never above the cost
+    # against the centers before the update)
+    Cp = C0
+    for it in range(4):
+        Cn, lab_it, inert_it, _ = on.lloyd(X, C0, it + 1, 0.0)
+        np.testing.assert_array_equal(lab_it, on.assign(X, Cp))
+        Cm, _ = on.update_centers(X, lab_it, Cp)
+        np.testing.assert_array_equal(Cn, Cm)
+        cost = on.blocked_sum(_dist_to_own_center(X, Cn, lab_it))
+        assert inert_it[-1] == cost
+        assert cost <= on.blocked_sum(_dist_to_own_center(X, Cp, lab_it))
+        Cp = Cn
