@@ -1,13 +1,33 @@
-"""VAMP: GPU lagged moments (C00, C0t, Ctt, separate means), host FP64 SVD, GPU projection.
+"""VAMP on the GPU moments: covariances from one device pass, FP64 host algebra.
 
-Mirrors pyemma/coordinates/transform/vamp.py:39-1006.  The three covariance
-matrices come from one non-reversible pass of the moments kernels
-(`emma_moments_lagged` with C0T | CTT and REMOVE_MEAN, the seam
-`LaggedCovariance(c00, c0t, ctt, remove_data_mean=True, reversible=False)` of
-vamp.py:578-596); the Koopman SVD, scores and expectations are d x d host
-FP64 algebra, as in the reference; the projections run on the GPU
-(`emma_project`).  The Chapman-Kolmogorov validator (vamp.py:839-952,
-1009-1101) builds on the MSM package and is not part of this path.
+Drop-in for pyemma.coordinates.vamp / VAMP (pyemma/coordinates/transform/
+vamp.py:370-1006, API api.py:1294-1443): same arguments, attributes, scores
+and projections.  The three covariances C00, C0t, Ctt with separate means
+come from one non-reversible pass of `emma_moments_lagged` (flags C0T | CTT
+| REMOVE_MEAN: the seam LaggedCovariance(c00, c0t, ctt,
+remove_data_mean=True, reversible=False) of vamp.py:578-596); projections
+run on the GPU (`emma_project`).  The Chapman-Kolmogorov validator
+(vamp.py:839-952, 1009-1101) builds on the MSM package and is out of scope.
+
+The host algebra follows the published method (H. Wu and F. Noe,
+"Variational approach for learning Markov processes from time series data",
+J. Nonlinear Sci. 30, 23-66 (2020), sections 3-4):
+
+* whitening: Lx with Lx^T C00 Lx = I and Ly with Ly^T Ctt Ly = I on the
+  eigenspaces of C00 and Ctt above epsilon (ranks rx, ry);
+* the half-weighted Koopman matrix Kw = Lx^T C0t Ly = A diag(s) B^T (SVD);
+* left singular functions psi(x) = U^T (x - mean_0) with U = Lx A, right
+  ones phi(y) = V^T (y - mean_t) with V = Ly B, truncated to the output
+  dimension m (optionally scaled by s, the kinetic map);
+* VAMP-r score of the m singular functions on covariances (C00', C0t',
+  Ctt') (the model's own, or a test set's):
+      sum_i t_i^r + 1,  t = singular values of
+      (U^T C00' U)^-1/2 (U^T C0t' V) (V^T Ctt' V)^-1/2   (r = 1, 2);
+  VAMP-E: 2 tr(S U^T C0t' V) - tr(S U^T C00' U S V^T Ctt' V) + 1, S = diag(s);
+  the 1 is the constant singular function (singular value 1);
+* propagation: on the bases [1, psi] of the present and [1, phi] of the
+  future the model is diag(1, s); n lag times chain it through the overlap
+  of the future basis with the present one.
 """
 import warnings
 
@@ -19,217 +39,210 @@ from .._solvers import _bounded_threads, spd_inv_split, spd_inv_sqrt
 from ._data import DeviceTrajectories, to_device
 from .covariance import LaggedCovariance
 
-__all__ = ["VAMPModel", "VAMP"]
+__all__ = ["VAMPModel", "VAMP", "output_dimension"]
+
+_SCALINGS = (None, "km", "kinetic map")
+_SCORES = ("VAMP1", "VAMP2", "VAMPE")
 
 
-def _mdot(*args):
-    """pyemma/util/linalg.py:30-40: chained matrix product."""
-    out = args[0]
-    for a in args[1:]:
-        out = np.dot(out, a)
-    return out
+def output_dimension(rank_left, rank_right, dim, singular_values):
+    """Number of singular functions kept (the rule of vamp.py:128-136): all
+    (min of the two whitening ranks) for dim None or 1.0, the smallest count
+    whose cumulative kinetic variance reaches a float dim, else min(ranks, dim)."""
+    if dim is None or (isinstance(dim, float) and dim == 1.0):
+        return min(rank_left, rank_right)
+    if isinstance(dim, float):
+        return int(np.searchsorted(kinetic_variance(singular_values), dim)) + 1
+    return int(min(rank_left, rank_right, dim))
+
+
+def kinetic_variance(singular_values):
+    """Cumulative share of sum s_i^2 carried by the leading singular values."""
+    c = np.cumsum(np.asarray(singular_values) ** 2)
+    return c / c[-1]
+
+
+def _whitener(C, epsilon):
+    """L with L^T C L = I on the eigenspace of C above epsilon, and its rank."""
+    L = np.atleast_2d(spd_inv_split(C, epsilon=epsilon))
+    return L, L.shape[1]
+
+
+class _Svd(object):
+    """Result of one decomposition of the covariances."""
+    __slots__ = ("s", "U", "V", "rank_left", "rank_right", "m")
+
+    def __init__(self, s, U, V, rank_left, rank_right, m):
+        self.s, self.U, self.V = s, U, V
+        self.rank_left, self.rank_right, self.m = rank_left, rank_right, m
 
 
 class VAMPModel(object):
-    """vamp.py:39-367: covariances -> half-weighted Koopman matrix -> SVD."""
+    """Covariances and means of a VAMP estimate, decomposed on first use.
 
-    def __init__(self, mean_0=None, mean_t=None, C00=None, Ctt=None, C0t=None, dim=None, epsilon=1e-6, scaling=None):
-        self.set_model_params(mean_0=mean_0, mean_t=mean_t, C00=C00, Ctt=Ctt, C0t=C0t, dim=dim, epsilon=epsilon,
-                              scaling=scaling)
+    Every field the decomposition depends on (covariances, dim, epsilon,
+    scaling) drops the cached decomposition when it is assigned.
+    """
 
-    def set_model_params(self, mean_0, mean_t, C00, Ctt, C0t, dim, epsilon, scaling=None):
-        self.mean_0 = mean_0
-        self.mean_t = mean_t
-        self.C00 = C00
-        self.Ctt = Ctt
-        self.C0t = C0t
-        self._svd_performed = False
-        self.dim = dim
-        self.epsilon = epsilon
-        self.scaling = scaling
+    _INPUTS = frozenset(("C00", "C0t", "Ctt", "dim", "epsilon", "scaling"))
 
-    def update_model_params(self, **kw):
-        for k, v in kw.items():
-            setattr(self, k, v)
+    def __init__(self, mean_0=None, mean_t=None, C00=None, Ctt=None, C0t=None, dim=None, epsilon=1e-6,
+                 scaling=None):
+        self._svd = None
+        self.set_model_params(mean_0=mean_0, mean_t=mean_t, C00=C00, Ctt=Ctt, C0t=C0t, dim=dim,
+                              epsilon=epsilon, scaling=scaling)
 
-    @property
-    def scaling(self):
-        return self._scaling
-
-    @scaling.setter
-    def scaling(self, value):
-        if value not in (None, "km", "kinetic map"):
+    def __setattr__(self, name, value):
+        if name == "scaling" and value not in _SCALINGS:
             raise ValueError('unexpected value (%s) of "scaling". Must be one of ("km", "kinetic map", None).'
                              % value)
-        self._scaling = value
+        if name in self._INPUTS:
+            object.__setattr__(self, "_svd", None)
+        object.__setattr__(self, name, value)
 
-    # the covariance setters invalidate the decomposition (vamp.py:90-116)
-    @property
-    def C00(self):
-        return self._C00
+    def set_model_params(self, mean_0, mean_t, C00, Ctt, C0t, dim, epsilon, scaling=None):
+        for name, value in (("mean_0", mean_0), ("mean_t", mean_t), ("C00", C00), ("Ctt", Ctt), ("C0t", C0t),
+                            ("dim", dim), ("epsilon", epsilon), ("scaling", scaling)):
+            setattr(self, name, value)
 
-    @C00.setter
-    def C00(self, val):
-        self._svd_performed = False
-        self._C00 = val
+    def update_model_params(self, **params):
+        for name, value in params.items():
+            setattr(self, name, value)
 
-    @property
-    def C0t(self):
-        return self._C0t
+    # ---- decomposition ----
+    def _decompose(self):
+        with _bounded_threads():
+            Lx, rx = _whitener(self.C00, self.epsilon)
+            Ly, ry = _whitener(self.Ctt, self.epsilon)
+            A, s, Bt = sla.svd(Lx.T.dot(self.C0t).dot(Ly), compute_uv=True, lapack_driver="gesvd")
+        m = output_dimension(rx, ry, self.dim, s)
+        U = Lx.dot(A[:, :m])
+        V = Ly.dot(Bt[:m].T)
+        if self.scaling is not None:  # kinetic map: singular functions scaled by their singular values
+            U = U * s[:m]
+            V = V * s[:m]
+        return _Svd(s, U, V, rx, ry, m)
 
-    @C0t.setter
-    def C0t(self, val):
-        self._svd_performed = False
-        self._C0t = val
-
-    @property
-    def Ctt(self):
-        return self._Ctt
-
-    @Ctt.setter
-    def Ctt(self, val):
-        self._svd_performed = False
-        self._Ctt = val
+    def _current(self):
+        if self._svd is None:
+            object.__setattr__(self, "_svd", self._decompose())
+        return self._svd
 
     @property
     def U(self):
-        if not self._svd_performed:
-            self._diagonalize()
-        return self._U
+        return self._current().U
 
     @property
     def V(self):
-        if not self._svd_performed:
-            self._diagonalize()
-        return self._V
+        return self._current().V
 
     @property
     def singular_values(self):
-        if not self._svd_performed:
-            self._diagonalize()
-        return self._singular_values
-
-    @staticmethod
-    def _cumvar(singular_values):
-        cumvar = np.cumsum(singular_values ** 2)
-        cumvar /= cumvar[-1]
-        return cumvar
+        return self._current().s
 
     @property
     def cumvar(self):
-        return VAMPModel._cumvar(self.singular_values)
-
-    @staticmethod
-    def _dimension(rank0, rankt, dim, singular_values):
-        """vamp.py:128-136."""
-        if dim is None or (isinstance(dim, float) and dim == 1.0):
-            return min(rank0, rankt)
-        if isinstance(dim, float):
-            return np.searchsorted(VAMPModel._cumvar(singular_values), dim) + 1
-        return np.min([rank0, rankt, dim])
+        return kinetic_variance(self.singular_values)
 
     def dimension(self):
-        """vamp.py:138-148."""
+        """Output dimension (vamp.py:138-148): the user's integer dim before any estimate."""
         if self.C00 is None:
             if isinstance(self.dim, int):
                 warnings.warn("Returning user-input for dimension, since this model has not yet been estimated.")
                 return self.dim
             raise RuntimeError("Please call set_model_params prior using this method.")
-        if not self._svd_performed:
-            self._diagonalize()
-        return self._dimension(self._rank0, self._rankt, self.dim, self.singular_values)
+        return self._current().m
 
-    def _diagonalize(self):
-        """vamp.py:264-301: W = L0^T C0t Lt with C00^-1 = L0 L0^T, Ctt^-1 = Lt Lt^T; SVD (gesvd)."""
-        with _bounded_threads():
-            L0 = spd_inv_split(self.C00, epsilon=self.epsilon)
-            self._rank0 = L0.shape[1] if L0.ndim == 2 else 1
-            Lt = spd_inv_split(self.Ctt, epsilon=self.epsilon)
-            self._rankt = Lt.shape[1] if Lt.ndim == 2 else 1
-            W = np.dot(L0.T, self.C0t).dot(Lt)
-            A, s, BT = sla.svd(W, compute_uv=True, lapack_driver="gesvd")
-            self._singular_values = s
-            m = VAMPModel._dimension(self._rank0, self._rankt, self.dim, self._singular_values)
-            U = np.dot(L0, A[:, :m])
-            V = np.dot(Lt, BT[:m, :].T)
-        if self.scaling is not None:
-            U *= s[np.newaxis, 0:m]
-            V *= s[np.newaxis, 0:m]
-        self._U = U
-        self._V = V
-        self._svd_performed = True
+    # ---- propagation ----
+    def _future_coefficients(self, observables, m, mean_free):
+        """Coefficients of linear observables g(y) = G^T y on [1, phi_1..m]."""
+        G = np.asarray(observables, dtype=np.float64)
+        coef = np.empty((G.shape[1], m + 1))
+        coef[:, 0] = 0.0 if mean_free else G.T.dot(self.mean_t)
+        coef[:, 1:] = G.T.dot(self.Ctt).dot(self.V[:, :m])
+        return coef
+
+    def _present_coefficients(self, statistics, m, mean_free):
+        """Coefficients of linear statistics f(x) = F^T x on [1, psi_1..m]."""
+        F = np.asarray(statistics, dtype=np.float64)
+        coef = np.empty((F.shape[1], m + 1))
+        coef[:, 0] = 0.0 if mean_free else F.T.dot(self.mean_0)
+        coef[:, 1:] = F.T.dot(self.C00).dot(self.U[:, :m])
+        return coef
+
+    def _propagator(self, m, steps):
+        """(m+1) x (m+1) model of `steps` lag times between the future basis [1, phi]
+        and the present basis [1, psi]: diag(1, s) once, and between steps the
+        overlap of the future basis with the present one, [1, psi](y) =
+        O [1, phi](y) with O[1:, 0] = U^T (mean_t - mean_0), O[1:, 1:] = U^T Ctt V."""
+        sig = np.diag(np.concatenate(([1.0], self.singular_values[:m])))
+        if steps == 1:
+            return sig
+        U, V = self.U[:, :m], self.V[:, :m]
+        overlap = np.eye(m + 1)
+        overlap[1:, 0] = U.T.dot(self.mean_t - self.mean_0)
+        overlap[1:, 1:] = U.T.dot(self.Ctt).dot(V)
+        return np.linalg.matrix_power(sig.dot(overlap), steps - 1).dot(sig)
 
     def expectation(self, observables, statistics, lag_multiple=1, observables_mean_free=False,
                     statistics_mean_free=False):
-        """vamp.py:150-262: future expectation / lagged covariance of linear observables."""
-        dim = self.dimension()
-        S = np.diag(np.concatenate(([1.0], self.singular_values[0:dim])))
-        V = self.V[:, 0:dim]
-        U = self.U[:, 0:dim]
-        m_0 = self.mean_0
-        m_t = self.mean_t
-        assert lag_multiple >= 1, "lag_multiple = 0 not implemented"
-        if lag_multiple == 1:
-            P = S
-        else:
-            p = np.zeros((dim + 1, dim + 1))
-            p[0, 0] = 1.0
-            p[1:, 0] = U.T.dot(m_t - m_0)
-            p[1:, 1:] = U.T.dot(self.Ctt).dot(V)
-            P = np.linalg.matrix_power(S.dot(p), lag_multiple - 1).dot(S)
-        Q = np.zeros((observables.shape[1], dim + 1))
-        if not observables_mean_free:
-            Q[:, 0] = observables.T.dot(m_t)
-        Q[:, 1:] = observables.T.dot(self.Ctt).dot(V)
-        if statistics is not None:
-            R = np.zeros((statistics.shape[1], dim + 1))
-            if not statistics_mean_free:
-                R[:, 0] = statistics.T.dot(m_0)
-            R[:, 1:] = statistics.T.dot(self.C00).dot(U)
-            return Q.dot(P).dot(R.T)
-        return Q.dot(P)[:, 0]
+        """Model prediction of E[g(x_{t + n tau})] (statistics None) or of the
+        lagged covariances E[g(x_{t + n tau}) f(x_t)^T] for linear observables
+        g and statistics f (the quantities of vamp.py:150-262)."""
+        if lag_multiple < 1:
+            raise AssertionError("lag_multiple = 0 not implemented")
+        m = self.dimension()
+        out = self._future_coefficients(observables, m, observables_mean_free).dot(
+            self._propagator(m, int(lag_multiple)))
+        if statistics is None:
+            return out[:, 0]
+        return out.dot(self._present_coefficients(statistics, m, statistics_mean_free).T)
 
+    # ---- scores ----
     def score(self, test_model=None, score_method="VAMP2"):
-        """vamp.py:303-367: VAMP1 / VAMP2 / VAMPE (+1 for the constant singular function)."""
-        if test_model is None:
-            test_model = self
-        Uk = self.U[:, 0:self.dimension()]
-        Vk = self.V[:, 0:self.dimension()]
-        res = None
+        """VAMP-1, VAMP-2 or VAMP-E score of this model's singular functions on the
+        covariances of test_model (default: this model), +1 for the constant
+        function (vamp.py:303-367)."""
+        if score_method not in _SCORES:
+            raise ValueError('"score" should be one of VAMP1, VAMP2 or VAMPE')
+        data = self if test_model is None else test_model
+        m = self.dimension()
+        U, V = self.U[:, :m], self.V[:, :m]
         with _bounded_threads():
-            if score_method == "VAMP1" or score_method == "VAMP2":
-                A = spd_inv_sqrt(Uk.T.dot(test_model.C00).dot(Uk))
-                B = Uk.T.dot(test_model.C0t).dot(Vk)
-                C = spd_inv_sqrt(Vk.T.dot(test_model.Ctt).dot(Vk))
-                ABC = _mdot(A, B, C)
-                if score_method == "VAMP1":
-                    res = np.linalg.norm(ABC, ord="nuc")
-                else:
-                    res = np.linalg.norm(ABC, ord="fro") ** 2
-            elif score_method == "VAMPE":
-                Sk = np.diag(self.singular_values[0:self.dimension()])
-                res = np.trace(2.0 * _mdot(Vk, Sk, Uk.T, test_model.C0t)
-                               - _mdot(Vk, Sk, Uk.T, test_model.C00, Uk, Sk, Vk.T, test_model.Ctt))
+            cross = U.T.dot(data.C0t).dot(V)
+            if score_method == "VAMPE":
+                s = self.singular_values[:m]
+                gram_x = U.T.dot(data.C00).dot(U)
+                gram_y = V.T.dot(data.Ctt).dot(V)
+                value = 2.0 * np.dot(s, np.diag(cross)) - np.trace((s[:, None] * gram_x).dot(s[:, None] * gram_y))
             else:
-                raise ValueError('"score" should be one of VAMP1, VAMP2 or VAMPE')
-        assert res
-        return res + 1
+                wx = np.atleast_2d(spd_inv_sqrt(U.T.dot(data.C00).dot(U)))
+                wy = np.atleast_2d(spd_inv_sqrt(V.T.dot(data.Ctt).dot(V)))
+                t = np.linalg.svd(wx.dot(cross).dot(wy), compute_uv=False)
+                value = np.sum(t) if score_method == "VAMP1" else np.sum(t ** 2)
+        return float(value) + 1.0
+
+
+def _model_param(name):
+    """Estimator attribute stored on its model (vamp.py:531-576)."""
+    return property(lambda self: getattr(self._model, name), lambda self, v: setattr(self._model, name, v))
 
 
 class VAMP(object):
     """Variational approach for Markov processes (vamp.py:370-1006)."""
 
+    dim = _model_param("dim")
+    epsilon = _model_param("epsilon")
+    scaling = _model_param("scaling")
+
     def __init__(self, lag, dim=None, scaling=None, right=False, epsilon=1e-6, stride=1, skip=0,
                  ncov_max=float("inf"), precision="fp64"):
-        self._model = VAMPModel()
+        self._model = VAMPModel(dim=dim, epsilon=epsilon, scaling=scaling)
         self._covar = None
         self._data = None
         self._estimated = False
         self.lag = int(lag)
-        self.dim = dim
-        self.scaling = scaling
         self.right = right
-        self.epsilon = epsilon
         self.stride = stride
         self.skip = skip
         self.ncov_max = ncov_max
@@ -239,78 +252,52 @@ class VAMP(object):
     def describe(self):
         return "[VAMP, lag = %i; max. output dim. = %s]" % (self.lag, str(self.dim))
 
-    # parameters live on the model (vamp.py:531-576)
     @property
     def model(self):
         return self._model
 
-    @property
-    def dim(self):
-        return self._model.dim
-
-    @dim.setter
-    def dim(self, value):
-        self._model.dim = value
-
-    @property
-    def epsilon(self):
-        return self._model.epsilon
-
-    @epsilon.setter
-    def epsilon(self, value):
-        self._model.epsilon = value
-
-    @property
-    def scaling(self):
-        return self._model.scaling
-
-    @scaling.setter
-    def scaling(self, value):
-        self._model.scaling = value
-
-    def _new_covar(self):
-        """vamp.py:578-596: C00, C0t, Ctt of the non-reversible, data-mean-free estimator."""
+    def _estimator(self):
         return LaggedCovariance(c00=True, c0t=True, ctt=True, remove_data_mean=True, reversible=False, lag=self.lag,
                                 bessel=False, stride=self.stride, skip=self.skip, weights=None,
                                 ncov_max=self.ncov_max, precision=self.precision)
 
-    def _update_model(self, covar):
-        self._model.update_model_params(mean_0=covar.mean, mean_t=covar.mean_tau, C00=covar.C00_, C0t=covar.C0t_,
-                                        Ctt=covar.Ctt_)
+    def _take(self, covar):
+        self._model.update_model_params(mean_0=covar.mean, mean_t=covar.mean_tau, C00=covar.C00_,
+                                        C0t=covar.C0t_, Ctt=covar.Ctt_)
 
-    def estimate(self, X, **kwargs):
-        """vamp.py:604-623."""
-        for k, v in kwargs.items():
-            if k == "chunksize":
-                self.chunksize = v
-            elif hasattr(self, k):
-                setattr(self, k, v)
-        dt = to_device(X)
-        if isinstance(self.dim, int) and not self.dim <= dt.d:
+    def _check_dim(self, dt):
+        if isinstance(self.dim, int) and self.dim > dt.d:
             raise RuntimeError("requested more output dimensions (%i) than dimension of input data (%i)"
                                % (self.dim, dt.d))
-        covar = self._new_covar()
+
+    def estimate(self, X, **kwargs):
+        """vamp.py:604-623: one moments pass, then the decomposition."""
+        for name, value in kwargs.items():
+            if name == "chunksize":
+                self.chunksize = value
+            elif hasattr(self, name):
+                setattr(self, name, value)
+        dt = to_device(X)
+        self._check_dim(dt)
+        covar = self._estimator()
         covar.estimate(dt)
         self._data = dt
-        self._update_model(covar)
-        self._model._diagonalize()
+        self._take(covar)
+        self._model.singular_values  # decompose now, as the reference's estimate does
         self._estimated = True
         return self
 
-    def fit(self, X, **kwargs):
-        return self.estimate(X, **kwargs)
+    fit = estimate
 
     def partial_fit(self, X):
-        """vamp.py:625-655: the decomposition is deferred to the first access."""
+        """vamp.py:625-655: moments accumulate; the decomposition waits for the next access."""
         dt = to_device(X)
-        if isinstance(self.dim, int) and not self.dim <= dt.d:
-            raise RuntimeError("requested more output dimensions (%i) than dimension of input data (%i)"
-                               % (self.dim, dt.d))
+        self._check_dim(dt)
         if self._covar is None:
-            self._covar = self._new_covar()
+            self._covar = self._estimator()
         self._covar.partial_fit(dt)
         if self._covar._estimated:
-            self._update_model(self._covar)
+            self._take(self._covar)
         self._estimated = False
         return self
 
@@ -339,33 +326,30 @@ class VAMP(object):
     def expectation(self, observables, statistics, lag_multiple=1, observables_mean_free=False,
                     statistics_mean_free=False):
         return self._model.expectation(observables, statistics, lag_multiple=lag_multiple,
-                                       statistics_mean_free=statistics_mean_free,
-                                       observables_mean_free=observables_mean_free)
+                                       observables_mean_free=observables_mean_free,
+                                       statistics_mean_free=statistics_mean_free)
 
     def score(self, test_data=None, score_method="VAMP2"):
-        """vamp.py:953-1006: self score, or cross-validation against a model of test_data with the same
-        hyperparameters."""
+        """Self score, or the score of this model on a VAMP estimate of test_data
+        with the same hyperparameters (vamp.py:953-1006)."""
         if test_data is None:
             return self._model.score(None, score_method=score_method)
-        est = VAMP(self.lag, dim=self.dim, scaling=self.scaling, right=self.right, epsilon=self.epsilon,
-                   stride=self.stride, skip=self.skip, ncov_max=self.ncov_max, precision=self.precision)
-        est.estimate(test_data)
-        return self._model.score(est.model, score_method=score_method)
+        other = VAMP(self.lag, dim=self.dim, scaling=self.scaling, right=self.right, epsilon=self.epsilon,
+                     stride=self.stride, skip=self.skip, ncov_max=self.ncov_max, precision=self.precision)
+        other.estimate(test_data)
+        return self._model.score(other.model, score_method=score_method)
 
-    # ---- projection (vamp.py:661-687) on the GPU ----
+    # ---- projection on the GPU (vamp.py:661-687) ----
     def _project_device(self, dt):
         m = int(self.dimension())
-        if self.right:
-            mean, M = self._model.mean_t, self._model.V[:, 0:m]
-        else:
-            mean, M = self._model.mean_0, self._model.U[:, 0:m]
+        mean, M = ((self._model.mean_t, self._model.V) if self.right else (self._model.mean_0, self._model.U))
         t = _native.torch()
         mean_d = t.as_tensor(np.ascontiguousarray(mean, dtype=np.float64), device=dt.X.device)
-        M_d = t.as_tensor(np.ascontiguousarray(M, dtype=np.float64), device=dt.X.device)
+        M_d = t.as_tensor(np.ascontiguousarray(M[:, :m], dtype=np.float64), device=dt.X.device)
         return _native.project(dt.X, mean_d, M_d, m)
 
     def transform(self, X):
-        single = isinstance(X, np.ndarray) or (not isinstance(X, (list, tuple, DeviceTrajectories)))
+        single = not isinstance(X, (list, tuple, DeviceTrajectories))
         dt = to_device(X)
         out = dt.split(self._project_device(dt))
         return out[0] if single and dt.ntraj == 1 else out

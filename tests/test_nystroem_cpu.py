@@ -88,9 +88,9 @@ def test_bordered_inverse_matches_pinv(case):
     oasis = oASIS_Nystroem(d, C0[:, cols[:1]], cols[:1])
     oasis.add_columns(C0[:, cols[1:]], cols[1:])
     np.testing.assert_allclose(oasis.Wk_inv, np.linalg.inv(C0[np.ix_(cols, cols)]), rtol=1e-8)
-    R = oasis._R_k.copy()
+    R = oasis.Rk.copy()
     oasis.update_inverse()
-    np.testing.assert_allclose(R, oasis._R_k, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(R, oasis.Rk, rtol=1e-8, atol=1e-10)
 
 
 def test_constant_evec():

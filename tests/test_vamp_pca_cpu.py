@@ -13,7 +13,7 @@ import pytest
 
 from oracle.moments import covariance_lagged as oracle_cov
 from pyemma_amd.coordinates.pca import PCA
-from pyemma_amd.coordinates.vamp import VAMPModel
+from pyemma_amd.coordinates.vamp import VAMPModel, output_dimension
 
 T3 = np.array([[0.7, 0.2, 0.1], [0.1, 0.8, 0.1], [0.1, 0.1, 0.8]])
 
@@ -126,13 +126,13 @@ def test_vamp_dimension_rules():
         VAMPModel(dim=None).dimension()
     with pytest.raises(ValueError):
         VAMPModel(scaling="bogus")
-    assert VAMPModel._dimension(4, 3, None, np.ones(3)) == 3
-    assert VAMPModel._dimension(4, 3, 2, np.ones(3)) == 2
+    assert output_dimension(4, 3, None, np.ones(3)) == 3
+    assert output_dimension(4, 3, 2, np.ones(3)) == 2
     # float dim: searchsorted into the cumulative kinetic variance
     sv = np.array([0.9, 0.5, 0.1])
     cv = np.cumsum(sv ** 2) / np.sum(sv ** 2)
-    assert VAMPModel._dimension(3, 3, 0.5, sv) == np.searchsorted(cv, 0.5) + 1 == 1
-    assert VAMPModel._dimension(3, 3, 0.9, sv) == 2
+    assert output_dimension(3, 3, 0.5, sv) == np.searchsorted(cv, 0.5) + 1 == 1
+    assert output_dimension(3, 3, 0.9, sv) == 2
 
 
 def test_vamp_kinetic_map_scaling(msm_case):
