@@ -27,23 +27,56 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_d300_k1000.md")
 
 
+KERNEL_SOURCES = {"moments": ["moments.hip", "emma_common.hpp"], "assign": ["assign.hip", "emma_common.hpp"]}
+
+
+def source_digest(kernel):
+    """sha256 (first 16 hex digits) of the HIP sources a kernel is built from."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES[kernel]:
+        with open(os.path.join(ROOT, "pyemma_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel, default_workload):
     """HBM bytes per call of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary
-    (calibrated against a plain copy), or None when the summary does not cover it or
-    the run is not the profiled workload."""
+    (calibrated against a plain copy).  None when the summary does not cover it, when
+    the run is not the profiled workload, or when the kernel's sources changed since
+    the summary was taken (its `sources:` line records their digest)."""
     if not default_workload:
         return None
+    vals, digests = {}, {}
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes:"):
-                    for item in line.split(":", 1)[1].split(";"):
-                        name, _, val = item.strip().partition("=")
-                        if name == kernel:
-                            return int(val)
+                for key, dst in (("traffic_bytes:", vals), ("sources:", digests)):
+                    if line.startswith(key):
+                        for item in line.split(":", 1)[1].split(";"):
+                            name, _, val = item.strip().partition("=")
+                            dst[name] = val
     except OSError:
         return None
-    return None
+    if kernel not in vals or digests.get(kernel) != source_digest(kernel):
+        return None
+    return int(vals[kernel])
+
+
+def mom_executed_flops(d, P):
+    """bf16 MFMA flops the reversible split moments kernel executes: the
+    64 x 64 blocks on and above the diagonal of the d x d result (blocks
+    wholly past d skipped), two matrices (UU, VV), four bf16 products (hh, hl,
+    lh, ll) per element, 2 flops per product and pair."""
+    nb = -(-d // 64)
+    return nb * (nb + 1) // 2 * 64 * 64 * 2 * 4 * 2 * float(P)
+
+
+def asg_executed_flops(d, k, n):
+    """bf16 MFMA flops of the split assignment: three products (hh, hl, lh)
+    over d padded to 16 and k padded to 32."""
+    return 3 * 2.0 * n * (-(-d // 16) * 16) * (-(-k // 32) * 32)
 
 
 def parse():
@@ -106,8 +139,9 @@ def main():
     ea = {k: torch.cuda.Event(enable_timing=True) for k in ("a0", "a1")}
 
     eig_s = []
+    last = {}
 
-    def step(prec=_native.EMMA_PREC_BF16X4_ACC, amode=_native.EMMA_ASSIGN_BF16X3_RECHECK):
+    def step(prec=_native.EMMA_PREC_BF16X4_ACC, amode=_native.EMMA_ASSIGN_BF16X3_RECHECK, labels=labels):
         # moments first (each kernel fills the GPU on its own), then the
         # assignment on the second stream while the host solves the d x d
         # eigenproblem of the moments
@@ -128,6 +162,7 @@ def main():
         eig_s.append(time.perf_counter() - te)
         torch.cuda.current_stream().wait_stream(s_asg)
         torch.cuda.synchronize()
+        last["C00"], last["C0t"] = C00, C0t
         return l, ev["m0"].elapsed_time(ev["m1"]), ea["a0"].elapsed_time(ea["a1"])
 
     for _ in range(args.warmup):
@@ -163,14 +198,20 @@ def main():
         % (step_s * 1e3, mom_ms, asg_ms, eig_ms, n_re))
 
     # the other moments modes, one call each on the same stream: FP64 tiles,
-    # and the bf16 split with single-level FP32 accumulation
-    for key, prec in (("moments_fp64", _native.EMMA_PREC_FP64), ("moments_bf16x4", _native.EMMA_PREC_BF16X4)):
+    # and the bf16 split with single-level FP32 accumulation; their matrices
+    # are kept for the eigenvalue comparison against the FP64 oracle
+    mats = {"bf16x4_acc": (last["C00"], last["C0t"])}
+    for key, name, prec in (("moments_fp64", "fp64", _native.EMMA_PREC_FP64),
+                            ("moments_bf16x4", "bf16x4", _native.EMMA_PREC_BF16X4)):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True, c0t=True, precision=prec)
+        w, _, _, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True,
+                                                      c0t=True, precision=prec)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1)
+        wh = w.item()
+        mats[name] = (Mxx.cpu().numpy() / wh, Mxy.cpu().numpy() / wh)
         kern[key] = {"ms": ms, "algorithmic_tflop": f_mom / 1e12, "tflops": f_mom / (ms * 1e-3) / 1e12}
         log("%s %.2f ms" % (key, ms))
 
@@ -180,19 +221,34 @@ def main():
     kern["kmeans"] = kmeans_leg(args, X, sub, cpu=not args.no_cpu)
 
     plain = None
+    labels_plain = None
     if not args.no_plain:
+        # the plain path writes its own label buffer: the headline labels stay
+        # untouched for the parity checks below
+        labels_plain = torch.empty_like(labels)
         t1 = time.perf_counter()
-        step(_native.EMMA_PREC_FP32, _native.EMMA_ASSIGN_FP32_PLAIN)
+        step(_native.EMMA_PREC_FP32, _native.EMMA_ASSIGN_FP32_PLAIN, labels=labels_plain)
         tp = time.perf_counter() - t1
         plain = {"value": N / tp, "unit": "frames/s", "ms_per_step": tp * 1e3,
                  "moments_ms": ev["m0"].elapsed_time(ev["m1"]), "assign_ms": ea["a0"].elapsed_time(ea["a1"]),
                  "kind": "FP32 MFMA moments + sequential FP32 assign loop"}
         log("plain GPU step %.1f ms" % (tp * 1e3))
 
+    parity = {}
+    if labels_plain is not None:
+        mism = int((labels != labels_plain).sum().item())
+        parity.update({"labels_identical_vs_plain_gpu": mism == 0, "label_mismatches_vs_plain_gpu": mism,
+                       "label_frames_vs_plain_gpu": N})
+        log("headline labels vs plain FP32 loop: %d mismatches over %d frames" % (mism, N))
+        del labels_plain
     cpu = None
-    parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(args, X, offsets, C, labels, l)
+        cpu, par = cpu_baseline(args, X, offsets, C, labels, mats)
+        parity.update(par)
+    kern["moments"]["executed_tflop"] = mom_executed_flops(d, P) / 1e12
+    kern["assign"]["executed_tflop"] = asg_executed_flops(d, args.k, N) / 1e12
+    ex = kern[dom]["executed_tflop"] * 1e12 / (kern[dom]["ms"] * 1e-3) / 1e12
+    roofline.update({"executed_tflops": ex, "executed_frac": ex / PEAK_BF16_TFLOPS})
 
     serving = None
     if args.serve_seconds > 0 or not args.no_hbm_legs:
@@ -471,57 +527,103 @@ def serving_leg(args):
     return out
 
 
-def cpu_baseline(args, X, offsets, C, labels_gpu, l_gpu):
-    """The oracle (CPU restatement of the reference path) on a bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(args, X, offsets, C, labels_gpu, mats, per_traj=500):
+    """The oracle (CPU restatement of the reference path) beside the GPU, and
+    the parity of the headline results against it.
+
+    Moments: the FP64 NumPy restatement over every trajectory of the
+    workload (not a sample), timed whole; its C00 and C0t are the FP64
+    reference for the eigenvalues of every GPU precision mode.  Assignment:
+    the C oracle with all host cores on `per_traj` frames drawn from each
+    trajectory (seeded), scaled to the N frames; those frames' headline GPU
+    labels are compared with it.  One-thread rates: the moments of one
+    trajectory with BLAS limited to one thread, the assignment of 2,000
+    frames with one OpenMP thread."""
     import torch
+    from threadpoolctl import threadpool_limits
 
     from oracle import eig as oeig
     from oracle import moments as om
     from oracle import native as on
-    from pyemma_amd import _native
+    from pyemma_amd import _solvers
 
     cores = len(os.sched_getaffinity(0))
     N, d = X.shape
-    # moments sample: whole trajectories
-    nt = max(1, args.cpu_frames // args.nframes)
-    sample = [X[offsets[i]:offsets[i + 1]].cpu().numpy() for i in range(nt)]
+    ntraj = len(offsets) - 1
     t0 = time.perf_counter()
-    ref = om.covariance_lagged(sample, args.lag, remove_data_mean=True, reversible=True, bessel=False)
-    t_mom = time.perf_counter() - t0
+    trajs = [X[offsets[i]:offsets[i + 1]].cpu().numpy() for i in range(ntraj)]
+    t_copy = time.perf_counter() - t0
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        ref = om.covariance_lagged(trajs, args.lag, remove_data_mean=True, reversible=True, bessel=False)
+        t_mom = time.perf_counter() - t0
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        om.covariance_lagged(trajs[:1], args.lag, remove_data_mean=True, reversible=True, bessel=False)
+        t_mom1 = (time.perf_counter() - t0) * N / len(trajs[0])
     t0 = time.perf_counter()
-    from pyemma_amd import _solvers
-    _solvers.eig_corr(ref["C00"], ref["C0t"], 1e-6, sign_maxelement=True)
+    l_ref, _ = _solvers.eig_corr(ref["C00"], ref["C0t"], 1e-6, sign_maxelement=True)
     t_eig = time.perf_counter() - t0
-    na = args.cpu_assign_frames
-    Xa = X[:na].cpu().numpy()
+    log("oracle FP64 moments of all %d trajectories: %.1f s (host copy %.1f s)" % (ntraj, t_mom, t_copy))
+    # assignment sample: per_traj frames of every trajectory
+    rng = np.random.default_rng(11)
+    idx = np.concatenate([np.sort(rng.choice(offsets[i + 1] - offsets[i], size=min(per_traj, offsets[i + 1]
+                                                                                  - offsets[i]), replace=False))
+                          + offsets[i] for i in range(ntraj)]).astype(np.int64)
+    idx_dev = torch.as_tensor(idx, device=X.device)
+    Xa = X[idx_dev].cpu().numpy()
     Ch = C.cpu().numpy()
     t0 = time.perf_counter()
     lab_ref = on.assign(Xa, Ch, nthreads=cores)
     t_asg = time.perf_counter() - t0
-    n_m = sum(len(s) for s in sample)
-    t_full = t_mom * N / n_m + t_eig + t_asg * N / na
-    cpu = {"value": N / t_full, "unit": "frames/s", "cores": cores, "kind": "port",
-           "sample": "moments: %d traj (%d frames, numpy FP64 dgemm restatement, %.2f s); eig d=%d (%.3f s); "
-                     "assign: %d frames x k=%d (C oracle, OpenMP %d threads, %.2f s); scaled to %d frames"
-                     % (nt, n_m, t_mom, d, t_eig, na, Ch.shape[0], cores, t_asg, N)}
-    # parity on the same sample: labels and eigenvalues of the GPU path
-    lab_gpu = labels_gpu[:na].cpu().numpy()
-    off_s = np.concatenate([[0], np.cumsum([len(s) for s in sample])]).astype(np.int64)
-    Xs = torch.as_tensor(np.concatenate(sample), device=X.device)
+    n1 = 2000
+    t0 = time.perf_counter()
+    on.assign(Xa[:n1], Ch, nthreads=1)
+    t_asg1 = time.perf_counter() - t0
+    na = len(idx)
+    t_full = t_mom + t_eig + t_asg * N / na
+    t_full1 = t_mom1 + t_eig + t_asg1 * N / n1
+    cpu = {"value": N / t_full, "unit": "frames/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+           "sample": "moments: all %d trajectories (%d frames, numpy FP64 dgemm restatement, %d BLAS threads, "
+                     "%.1f s, measured whole); eig d=%d (%.3f s); assign: %d frames (%d per trajectory) x k=%d "
+                     "(C oracle, OpenMP %d threads, %.2f s), scaled to %d frames"
+                     % (ntraj, N, cores, t_mom, d, t_eig, na, per_traj, Ch.shape[0], cores, t_asg, N),
+           "one_thread": {"value": N / t_full1, "unit": "frames/s",
+                          "sample": "moments of 1 trajectory with 1 BLAS thread, assign of %d frames with 1 "
+                                    "thread, scaled to %d frames" % (n1, N)}}
+    lab_gpu = labels_gpu[idx_dev].cpu().numpy()
+    mism = int(np.count_nonzero(lab_gpu != lab_ref))
+    parity = {"labels_identical_vs_oracle": mism == 0, "label_mismatches_vs_oracle": mism,
+              "label_frames_vs_oracle": na, "eig_frames": N,
+              "eig_reference": "FP64 oracle moments of all %d trajectories" % ntraj}
+    # every eigenvalue of the generalized problem, and every eigenvalue TICA
+    # returns (eig_corr with epsilon 1e-6: the same count is required)
     ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
-    parity = {"labels_identical": bool(np.array_equal(lab_gpu, lab_ref)), "label_frames": na, "eig_frames": n_m}
-    big = np.abs(ev_ref) >= 0.1
-    for name, prec in (("bf16x4", _native.EMMA_PREC_BF16X4), ("bf16x4_acc", _native.EMMA_PREC_BF16X4_ACC),
-                       ("fp64", _native.EMMA_PREC_FP64)):
-        w, sx, sy, Mxx, Mxy, _ = _native.moments_lagged(Xs, off_s, args.lag, reversible=True, remove_mean=True,
-                                                        c0t=True, precision=prec)
-        wh = w.item()
-        ev_gpu = oeig.generalized_eigenvalues(Mxx.cpu().numpy() / wh, Mxy.cpu().numpy() / wh)
-        rel = np.abs(ev_gpu - ev_ref) / np.abs(ev_ref)
+    parity["eig_count"] = int(len(ev_ref))
+    parity["eig_returned_count_oracle"] = int(len(l_ref))
+    for name, (C00, C0t) in mats.items():
+        ev = oeig.generalized_eigenvalues(C00, C0t)
+        rel = np.abs(ev - ev_ref) / np.abs(ev_ref)
         parity["eig_%s_rel_err_max" % name] = float(rel.max())
-        parity["eig_%s_rel_err_max_abs_ge_0.1" % name] = float(rel[big].max()) if big.any() else None
-        parity["eig_%s_abs_err_max" % name] = float(np.abs(ev_gpu - ev_ref).max())
+        parity["eig_%s_abs_err_max" % name] = float(np.abs(ev - ev_ref).max())
+        l_gpu, _ = _solvers.eig_corr(C00, C0t, 1e-6, sign_maxelement=True)
+        if len(l_gpu) == len(l_ref):
+            parity["eig_%s_returned_rel_err_max" % name] = float(np.max(np.abs(l_gpu - l_ref) / np.abs(l_ref)))
+        else:
+            parity["eig_%s_returned_count" % name] = int(len(l_gpu))
     log("cpu baseline %.3g frames/s on %d cores; parity %s" % (cpu["value"], cores, parity))
+    del trajs
     return cpu, parity
 
 

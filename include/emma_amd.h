@@ -254,6 +254,17 @@ int emma_count_transitions(const int32_t *dtraj, int64_t n, const int64_t *traj_
  */
 int emma_max_state(const int32_t *dtraj, int64_t n, int32_t *out, void *stream);
 
+/*
+ * D = A B + C by exactly one v_mfma_f32_32x32x16_bf16 instruction (one
+ * wave): A (32 x 16) and B (16 x 32) row-major bf16 bit patterns, C and D
+ * (32 x 32) row-major float32, all device memory.  No counterpart in the
+ * reference: it exposes the rounding of the matrix core's FP32
+ * accumulation, the property the error bound of EMMA_ASSIGN_BF16X3_RECHECK
+ * rests on (round to nearest, unit 2^-24), so that a test can check it on the
+ * device the library runs on.
+ */
+int emma_probe_mfma_bf16(const uint16_t *A, const uint16_t *B, const float *C, float *D, void *stream);
+
 #define EMMA_DIST_INVERSE 1u /* 1 / distance (InverseDistanceFeature) */
 
 /*

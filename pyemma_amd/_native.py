@@ -84,6 +84,8 @@ def lib():
     L.emma_pair_distances.argtypes = [vp, i64, i32, vp, i32, u32, vp, vp]
     L.emma_max_state.restype = ctypes.c_int
     L.emma_max_state.argtypes = [vp, i64, vp, vp]
+    L.emma_probe_mfma_bf16.restype = ctypes.c_int
+    L.emma_probe_mfma_bf16.argtypes = [vp, vp, vp, vp, vp]
     _lib = L
     return L
 

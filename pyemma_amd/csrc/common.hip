@@ -262,3 +262,34 @@ extern "C" int emma_row_affine(const float *X, int64_t n, int32_t d, const doubl
     k_row_affine<<<(unsigned)cdiv(n, 4), 256, 0, (hipStream_t)stream>>>(X, n, d, u, u0, out);
     return check_launch("row affine");
 }
+
+// ------------------------------------------------------------------
+// matrix-core rounding probe: one v_mfma_f32_32x32x16_bf16, one wave.
+// Operand layout of the instruction (lane l): A row l % 32, dims
+// 8 (l / 32) .. + 7; B column l % 32, the same dims; D column l % 32, rows
+// 8 (e / 4) + 4 (l / 32) + e % 4 for e = 0..15.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_probe_mfma_bf16(const uint16_t *__restrict__ A,
+                                                        const uint16_t *__restrict__ B,
+                                                        const float *__restrict__ C, float *__restrict__ D) {
+    const int lane = threadIdx.x;
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    emma::bf16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = __builtin_bit_cast(__bf16, A[r * 16 + kh + i]);
+        b[i] = __builtin_bit_cast(__bf16, B[(kh + i) * 32 + r]);
+    }
+    emma::floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = C[(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * 32 + r];
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) D[(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * 32 + r] = acc[e];
+}
+
+extern "C" int emma_probe_mfma_bf16(const uint16_t *A, const uint16_t *B, const float *C, float *D, void *stream) {
+    EMMA_REQUIRE(A && B && C && D, "emma_probe_mfma_bf16: NULL pointer");
+    k_probe_mfma_bf16<<<1, 64, 0, (hipStream_t)stream>>>(A, B, C, D);
+    return check_launch("mfma probe");
+}

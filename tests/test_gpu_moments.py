@@ -115,11 +115,16 @@ def test_no_pairs_gives_zero_weight():
 
 
 def test_tica_eigenvalues_vs_fp64():
-    """HMM data (config 1 generator, 100k x 30): eigenvalues vs the FP64 oracle.
+    """HMM data (the 100k x 30 4-state chain): every eigenvalue vs the FP64 oracle.
 
-    FP64 tiles: every eigenvalue within 1e-6 relative.  BF16X4 / FP32 tiles:
-    eigenvalues with |l| >= 0.1 within 1e-6 relative, all within 1e-5 absolute
-    (the near-zero noise eigenvalues carry the absolute FP32-class error).
+    FP64 tiles: every eigenvalue within 1e-6 relative.  The FP32-class modes
+    (bf16x4, fp32, bf16x4-acc): the three slow eigenvalues (|l| >= 0.1) within
+    1e-6 relative, and every eigenvalue within an absolute bar: 1e-6 for
+    bf16x4-acc, 1e-5 for the single-level modes.  The 26 noise eigenvalues of
+    this chain are of order 1e-3, so the absolute error of FP32-class
+    accumulation is a relative error of up to ~2e-4 on them (bf16x4-acc,
+    measured 1.7e-4); only fp64 mode holds 1e-6 relative on every eigenvalue
+    of such data.
     """
     from pyemma_amd.synthetic import hmm_trajectory
     X = hmm_trajectory(100000, 30, seed=1)
@@ -127,23 +132,28 @@ def test_tica_eigenvalues_vs_fp64():
     ref = om.covariance_lagged([X], lag, remove_data_mean=True, reversible=True, bessel=False)
     ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
     big = np.abs(ev_ref) >= 0.1
+    assert big.sum() == 3
     for prec in PRECS:
         w, sx, _, Mxx, Mxy, _ = _gpu([X], lag, prec, reversible=True, remove_mean=True, c0t=True)
         ev = oeig.generalized_eigenvalues(Mxx / w[0], Mxy / w[0])
         rel = np.abs(ev - ev_ref) / np.abs(ev_ref)
+        err = np.abs(ev - ev_ref)
+        print("prec %d: rel. err all %.3g, |l| >= 0.1 %.3g, abs. err all %.3g"
+              % (prec, rel.max(), rel[big].max(), err.max()))
         if prec == _native.EMMA_PREC_FP64:
             assert rel.max() < 1e-6, (prec, rel.max())
         else:
             assert rel[big].max() < 1e-6, (prec, rel[big].max())
-            assert np.abs(ev - ev_ref).max() < 1e-5, (prec, np.abs(ev - ev_ref).max())
+            assert err.max() < (1e-6 if prec == _native.EMMA_PREC_BF16X4_ACC else 1e-5), (prec, err.max())
 
 
 def test_accurate_mode_on_distance_features():
     """Pairwise distances of 25 Brownian particles (the 300 collinear features of the
     200 x 50,000 workload; 2 trajectories here), lag 20: the generalized eigenproblem
     amplifies the FP32 accumulation error of long MFMA chains.  BF16X4_ACC (two-level
-    FP32 accumulation) keeps the eigenvalues with |l| >= 0.1 within 1e-6 relative of
-    the FP64 oracle; plain BF16X4 is reported, with a loose bound."""
+    FP32 accumulation) keeps every one of the 300 eigenvalues within 1e-6 relative of
+    the FP64 oracle (on the whole 200-trajectory workload: 3.8e-9); plain BF16X4 is
+    held to 1e-4."""
     from pyemma_amd.synthetic import brownian_distances
     t = _native.torch()
     X, off = brownian_distances(2, 50000, seed=2, device=_native.device())
@@ -152,15 +162,15 @@ def test_accurate_mode_on_distance_features():
     trajs = [Xh[off[i]:off[i + 1]] for i in range(2)]
     ref = om.covariance_lagged(trajs, lag, remove_data_mean=True, reversible=True, bessel=False)
     ev_ref = oeig.generalized_eigenvalues(ref["C00"], ref["C0t"])
-    big = np.abs(ev_ref) >= 0.1
+    assert len(ev_ref) == 300
     errs = {}
     for prec in (_native.EMMA_PREC_BF16X4, _native.EMMA_PREC_BF16X4_ACC):
         w, sx, _, Mxx, Mxy, _ = [None if o is None else o.cpu().numpy() for o in _native.moments_lagged(
             X, off, lag, precision=prec, reversible=True, remove_mean=True, c0t=True)]
         t.cuda.synchronize()
         ev = oeig.generalized_eigenvalues(Mxx / w[0], Mxy / w[0])
-        errs[prec] = (np.abs(ev - ev_ref) / np.abs(ev_ref))[big].max()
-    print("eigenvalue rel. err (|l| >= 0.1): bf16x4 %.3g, bf16x4-acc %.3g"
+        errs[prec] = (np.abs(ev - ev_ref) / np.abs(ev_ref)).max()
+    print("eigenvalue rel. err (all 300): bf16x4 %.3g, bf16x4-acc %.3g"
           % (errs[_native.EMMA_PREC_BF16X4], errs[_native.EMMA_PREC_BF16X4_ACC]))
     assert errs[_native.EMMA_PREC_BF16X4_ACC] < 1e-6
     assert errs[_native.EMMA_PREC_BF16X4] < 1e-4
