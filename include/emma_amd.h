@@ -147,6 +147,28 @@ int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k,
                 void *stream);
 
 /*
+ * emma_assign without a workspace argument and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same labels and n_rechecked as emma_assign.  Its
+ * scratch (center images, recheck list) is a fixed device array of
+ * EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds
+ * until the process ends; frames are processed in chunks that fit it.
+ * Limit: the k centers' tile images, ceil(k/32) * (128 * (d_pad + 8) + 384)
+ * bytes with d_pad = d rounded up to the kernel's K-step width (32, 64, 128,
+ * 192, 256, 304, 384 or 512), must leave room for
+ * EMMA_ASSIGN_NOALLOC_MIN_CHUNK frames (8 bytes each): k <= 15,600 at
+ * d = 512, k <= 26,000 at d = 300; larger sets return EMMA_ERR_INVALID_ARG
+ * with a message.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (different streams, threads) race and must be serialized by the caller.
+ */
+#define EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define EMMA_ASSIGN_NOALLOC_MIN_CHUNK 65536
+int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const float *C, int32_t k, int32_t *labels,
+                        int32_t mode, int64_t *n_rechecked, void *stream);
+
+/*
  * Tile variants of the MFMA assignment kernel (waves per workgroup and
  * workgroups per CU), for the autotuner.  emma_assign_variants(d) is the
  * number of variants at dimension d (0 if d is beyond the MFMA path);

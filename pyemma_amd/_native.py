@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libemma_amd.so")
 
 EMMA_OK = 0
+EMMA_ERR_INVALID_ARG = -1
 EMMA_MOM_SYMMETRIZE = 1
 EMMA_MOM_REMOVE_MEAN = 2
 EMMA_MOM_C0T = 4
@@ -84,6 +85,8 @@ def lib():
     L.emma_pair_distances.argtypes = [vp, i64, i32, vp, i32, u32, vp, vp]
     L.emma_max_state.restype = ctypes.c_int
     L.emma_max_state.argtypes = [vp, i64, vp, vp]
+    L.emma_assign_noalloc.restype = ctypes.c_int
+    L.emma_assign_noalloc.argtypes = [vp, i64, i32, vp, i32, vp, i32, vp, vp]
     L.emma_probe_mfma_bf16.restype = ctypes.c_int
     L.emma_probe_mfma_bf16.argtypes = [vp, vp, vp, vp, vp]
     _lib = L

@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void k_assign_plain(const float *__restrict__ 
 }
 
 __global__ void k_copy_i64(int64_t *dst, const int64_t *src) { *dst = *src; }
+__global__ void k_add_i64(int64_t *dst, const int64_t *src) { *dst += *src; }
 
 // ------------------------------------------------------------------
 // center preparation for the MFMA path
@@ -1035,4 +1036,86 @@ extern "C" int emma_assign_select(int32_t d, int32_t k, int32_t variant) {
     }
     g_sel[ks_index(ks)][k_bucket((int)cdiv(k, 32) * 32)] = variant < 0 ? 0 : variant;
     return EMMA_OK;
+}
+
+// ------------------------------------------------------------------
+// allocation-free assignment: the workspace of emma_assign carved from a
+// file-scope device array instead of a caller buffer
+// ------------------------------------------------------------------
+namespace emma {
+namespace asg {
+
+// Every GPU the library runs on holds its own copy of this array until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (the array's name on the host
+// is not a device address).
+static __device__ __align__(256) uint8_t scratch[EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device, resolved once per
+// device (with that device current) and kept.
+static int scratch_on_current_device(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("emma_assign_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return EMMA_ERR_HIP;
+    }
+    EMMA_REQUIRE(dev >= 0 && dev < kMaxDevices, "emma_assign_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("emma_assign_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return EMMA_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return EMMA_OK;
+}
+
+}  // namespace asg
+}  // namespace emma
+
+extern "C" int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const float *C, int32_t k,
+                                   int32_t *labels, int32_t mode, int64_t *n_rechecked, void *stream) {
+    using namespace emma::asg;
+    EMMA_REQUIRE(n >= 0 && d > 0 && k > 0, "emma_assign_noalloc: bad sizes n=%lld d=%d k=%d", (long long)n, d, k);
+    EMMA_REQUIRE(mode == EMMA_ASSIGN_FP32_PLAIN || mode == EMMA_ASSIGN_BF16X3_RECHECK,
+                 "emma_assign_noalloc: unknown mode %d", mode);
+    if (n == 0) return EMMA_OK;
+    EMMA_REQUIRE(X && C && labels, "emma_assign_noalloc: NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == EMMA_ASSIGN_FP32_PLAIN)  // the plain loop needs no workspace
+        return assign_run(X, n, d, C, k, labels, mode, n_rechecked, nullptr, nullptr, s);
+    // the center images and constants take a fixed part of the scratch; the
+    // recheck list (8 bytes per frame) the rest, so the frames run in chunks
+    const size_t fixed = assign_workspace_bytes(0, d, k, mode);
+    constexpr size_t cap = EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES;
+    const int64_t chunk_max = fixed + 512 < cap ? (int64_t)((cap - fixed - 512) / 8) : 0;
+    EMMA_REQUIRE(chunk_max >= EMMA_ASSIGN_NOALLOC_MIN_CHUNK,
+                 "emma_assign_noalloc: k=%d centers of d=%d take %zu of the %zu scratch bytes; at most %lld bytes "
+                 "leave room for %d frames per chunk (use emma_assign with a workspace)",
+                 k, d, fixed, cap, (long long)(cap - 512 - 8 * (size_t)EMMA_ASSIGN_NOALLOC_MIN_CHUNK),
+                 EMMA_ASSIGN_NOALLOC_MIN_CHUNK);
+    const int64_t chunk = chunk_max < ((int64_t)1 << 22) ? chunk_max : ((int64_t)1 << 22);
+    EMMA_REQUIRE(assign_workspace_bytes(chunk, d, k, mode) <= cap, "emma_assign_noalloc: chunk of %lld frames "
+                 "exceeds the scratch", (long long)chunk);
+    void *ws = nullptr;
+    EMMA_TRY(scratch_on_current_device(&ws));
+    if (n_rechecked) k_zero_i64n<<<1, 64, 0, s>>>(n_rechecked, 1);
+    // the list length of each chunk is the first int64 of the carved workspace
+    const AssignWs w = carve_assign(ws, chunk, d, k, mode);
+    for (int64_t f0 = 0; f0 < n; f0 += chunk) {
+        const int64_t m = n - f0 < chunk ? n - f0 : chunk;
+        EMMA_TRY(assign_run(X + f0 * (int64_t)d, m, d, C, k, labels + f0, mode, nullptr, ws, nullptr, s));
+        if (n_rechecked) k_add_i64<<<1, 1, 0, s>>>(n_rechecked, w.count);
+    }
+    return check_launch("emma_assign_noalloc");
 }
