@@ -91,6 +91,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 GPU comparison")
     ap.add_argument("--no-hbm-legs", action="store_true", help="skip the small-d assignment legs")
+    ap.add_argument("--no-large-k", action="store_true", help="skip the 10^7 x 512, k = 4096 k-means leg")
     ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
     ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving leg (0: skip)")
@@ -251,11 +252,13 @@ def main():
     roofline.update({"executed_tflops": ex, "executed_frac": ex / PEAK_BF16_TFLOPS})
 
     serving = None
-    if args.serve_seconds > 0 or not args.no_hbm_legs:
+    if args.serve_seconds > 0 or not args.no_hbm_legs or not args.no_large_k:
         del X, sub, labels
         torch.cuda.empty_cache()
     if not args.no_hbm_legs:
         kern.update(assign_hbm_legs())
+    if not args.no_large_k:
+        kern["large_k"] = large_k_leg(args)
     if args.serve_seconds > 0:
         serving = serving_leg(args)
 
@@ -454,6 +457,84 @@ def assign_hbm_legs(n=10_000_000):
         log("assign d=%d k=%d: %.2f ms, %.0f GB/s (%.0f %% of HBM)" % (d, k, ms, gbs, 100 * gbs / PEAK_HBM_GBS))
         del X, C, lab
         torch.cuda.empty_cache()
+    return out
+
+
+def large_k_leg(args, n=10_000_000, d=512, k=4096, n_seed=200_000, iters=5, n_plain=1_000_000, n_cpu=20_000):
+    """k-means at the largest configured shape: 10^7 x 512 Gaussian-mixture
+    frames (64 components, means N(0, 4 I), diagonal variances log-uniform in
+    [0.1, 2]) generated on the device, kmeans++ with k = 4096 on a seeded
+    n_seed-frame subsample, the assignment of all frames to those centers,
+    and `iters` Lloyd iterations (tol 0) on all frames.  Parity: the first
+    n_plain headline labels against the plain FP32 loop on the device, and
+    n_cpu frames spread over the data against the oracle on the host."""
+    import torch
+
+    from pyemma_amd import _native, autotune
+    from pyemma_amd.synthetic import gaussian_mixture
+
+    t0 = time.time()
+    X = gaussian_mixture(n, d, seed=3, device=_native.device())
+    torch.cuda.synchronize()
+    log("large-k data %d x %d generated in %.1f s" % (n, d, time.time() - t0))
+    rng = np.random.default_rng(31)
+    sub = X[torch.as_tensor(np.sort(rng.choice(n, size=n_seed, replace=False)), device=X.device)].contiguous()
+    res = {}
+
+    def seed():
+        res["C"], _ = _native.kmeanspp(sub, k, 4242)
+
+    ms_pp = _timed(seed, reps=1)
+    C = res["C"]
+    variant = autotune.tune_assign(X, C)
+    lab = torch.empty(n, dtype=torch.int32, device=X.device)
+    nre = torch.zeros(1, dtype=torch.int64, device=X.device)
+    ms_asg = _timed(lambda: _native.assign(X, C, labels=lab, n_rechecked=nre))
+    n_re = int(nre.item())
+
+    def run_lloyd():
+        res["out"] = _native.lloyd(X, C.clone(), iters, 0.0)
+
+    ms_ll = _timed(run_lloyd, reps=1)
+    n_iter = int(res["out"][2].item())
+    f_asg = 2.0 * d * k * n
+    ex_asg = asg_executed_flops(d, k, n)
+    ach = f_asg / (ms_asg * 1e-3) / 1e12
+    pp_bytes = (4 * d + 8) * n_seed * (k - 1)
+    out = {"workload": "%d x %d f32 Gaussian mixture, k=%d" % (n, d, k),
+           "kmeanspp": {"frames": n_seed, "k": k, "ms": ms_pp, "bytes": pp_bytes,
+                        "gbs": pp_bytes / (ms_pp * 1e-3) / 1e9},
+           "assign": {"ms": ms_asg, "algorithmic_tflop": f_asg / 1e12, "tflops": ach,
+                      "executed_tflop": ex_asg / 1e12, "rechecked_frames": n_re, "tile_variant": variant,
+                      "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": ach / PEAK_BF16_TFLOPS,
+                                   "executed_frac": ex_asg / (ms_asg * 1e-3) / 1e12 / PEAK_BF16_TFLOPS}},
+           "lloyd": {"frames": n, "iterations": n_iter, "ms": ms_ll, "ms_per_iteration": ms_ll / max(n_iter, 1)}}
+    log("large-k: kmeans++ k=%d on %d frames %.0f ms; assign %.1f ms (%.0f TFLOP/s algorithmic, %d rechecked); "
+        "lloyd %d iterations %.0f ms" % (k, n_seed, ms_pp, ms_asg, ach, n_re, n_iter, ms_ll))
+    # parity of the headline labels
+    lab_plain = torch.empty(n_plain, dtype=torch.int32, device=X.device)
+    Xp = X[:n_plain]
+    ms_plain = _timed(lambda: _native.assign(Xp, C, mode=_native.EMMA_ASSIGN_FP32_PLAIN, labels=lab_plain), reps=1)
+    mism = int((lab[:n_plain] != lab_plain).sum().item())
+    out["parity"] = {"label_mismatches_vs_plain_gpu": mism, "label_frames_vs_plain_gpu": n_plain,
+                     "plain_gpu_ms_per_1e6_frames": ms_plain * 1e6 / n_plain}
+    if not args.no_cpu:
+        from oracle import native as on
+
+        idx = torch.as_tensor(np.sort(rng.choice(n, size=n_cpu, replace=False)), device=X.device)
+        cores = len(os.sched_getaffinity(0))
+        t1 = time.perf_counter()
+        ref = on.assign(X[idx].cpu().numpy(), C.cpu().numpy(), nthreads=cores)
+        t_cpu = time.perf_counter() - t1
+        cm = int(np.count_nonzero(lab[idx].cpu().numpy() != ref))
+        out["parity"].update({"label_mismatches_vs_oracle": cm, "label_frames_vs_oracle": n_cpu})
+        out["cpu"] = {"kind": "port", "cores": cores, "assign_ms_scaled": t_cpu * 1e3 * n / n_cpu,
+                      "sample": "assign of %d frames x k=%d (C oracle, %d threads, %.2f s), scaled to %d frames"
+                                % (n_cpu, k, cores, t_cpu, n)}
+    log("large-k parity: %s" % out["parity"])
+    del X, sub, lab, lab_plain, res
+    torch.cuda.empty_cache()
     return out
 
 
