@@ -253,6 +253,51 @@ __global__ void k_zero_i64n(int64_t *p, int n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_zero_i32n(int32_t *p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+// ------------------------------------------------------------------
+// the recheck of undecided frames
+// ------------------------------------------------------------------
+// Pass 1 (the MFMA kernel) appends every frame it cannot decide to `list`
+// together with `upper`, a rigorous upper bound (rounded up to FP32) of the
+// reference loop's distance to the frame's best candidate; NaN marks a
+// frame whose distances are not finite.  Pass 2 (the MFMA kernel in
+// candidate mode) recomputes the listed frames' distance bounds and records,
+// per list position p < slots, every center j whose rigorous lower bound
+// does not exceed upper[p] (at most CAND_MAX of them in cand[p][.], their
+// number in ccnt[p]); those centers contain the reference loop's choice.
+// Pass 3 runs the reference loop over the recorded candidates only; frames
+// beyond the slots, with more than CAND_MAX candidates or without a finite
+// bound go to `list2` and the full loop over every center.
+constexpr int CAND_MAX = 8;
+
+struct RecheckLists {
+    int64_t *list;   // [n] listed frames
+    int64_t *count;  // [4]: list length, bits of max |b_j|, list2 length, unused
+    float *upper;    // [n] upper bound per list position
+    int32_t *ccnt;   // [slots] candidates found per list position
+    int32_t *cand;   // [slots][CAND_MAX]
+    int64_t *list2;  // [n] frames for the full loop
+    int64_t slots;
+};
+
+// float >= v (v a finite double or NaN)
+__device__ __forceinline__ float f32_round_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = __uint_as_float(f >= 0.0f ? __float_as_uint(f) + 1u
+                                                   : (__float_as_uint(f) == 0x80000000u ? 1u : __float_as_uint(f) - 1u));
+    return f;
+}
+
+__device__ __forceinline__ void list_frame(const RecheckLists &rl, int64_t fr, float upper) {
+    const unsigned long long pos = atomicAdd((unsigned long long *)rl.count, 1ull);
+    rl.list[pos] = fr;
+    rl.upper[pos] = upper;
+}
+
 // ------------------------------------------------------------------
 // MFMA kernel
 // ------------------------------------------------------------------
@@ -300,7 +345,7 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 __device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fvalid, float tm, float t2, int imin,
                                                    int k, float na, float sa, float CE, float gref, float bmax,
                                                    const float *__restrict__ bn, int32_t *__restrict__ labels,
-                                                   int64_t *__restrict__ list, int64_t *__restrict__ list_count) {
+                                                   const RecheckLists &rl) {
     {
         const float ot = __shfl_xor(tm, 32, 64);
         const int oi = __shfl_xor(imin, 32, 64);
@@ -314,8 +359,7 @@ __device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fv
     }
     if (lane < 32 && fvalid && imin >= k) {
         labels[fr] = 0;  // non-finite distances: leave it to the exact loop
-        const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-        list[pos] = fr;
+        list_frame(rl, fr, __uint_as_float(0x7fc00000u));
     } else if (lane < 32 && fvalid) {
         const double bnb = (double)bn[imin];
         const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
@@ -323,10 +367,7 @@ __device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fv
         const double upper = ((double)na + (double)tm + E1) * (1.0 + (double)gref);
         const double lower = ((double)na + (double)t2 - Em) * (1.0 - (double)gref);
         labels[fr] = imin;
-        if (!(lower > upper)) {
-            const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-            list[pos] = fr;
-        }
+        if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
     }
 }
 
@@ -337,14 +378,22 @@ __device__ __forceinline__ int center_of_code(int code, int l) {
     return tt * 32 + 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
 }
 
-template <int KSTEPS, int NW, int MINB, int FB>
+// CAND: pass 2 of the recheck (candidate mode): the frames are the listed
+// ones at list positions below rl.slots, and the epilogue records the
+// centers whose lower bound reaches the frame's upper bound instead of
+// tracking the minimum.
+template <int KSTEPS, int NW, int MINB, int FB, bool CAND = false>
 __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float CE,
                                                             float gref, int32_t *__restrict__ labels,
-                                                            int64_t *__restrict__ list,
-                                                            int64_t *__restrict__ list_count,
-                                                            const int32_t *__restrict__ done) {
+                                                            RecheckLists rl, const int32_t *__restrict__ done) {
     if (done && *done) return;
+    int64_t nf = n;  // frames (pass 1) or listed frames with a candidate slot (pass 2)
+    if constexpr (CAND) {
+        const int64_t c = *rl.count;
+        nf = c < rl.slots ? c : rl.slots;
+        if ((int64_t)blockIdx.x * NW * FB * 32 >= nf) return;
+    }
     constexpr int DP = KSTEPS * 16;
     constexpr int PITCH = tile_pitch(DP);
     constexpr int IMG = tile_img_bytes(DP);
@@ -369,12 +418,16 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // FB blocks of 32 frames per wave: every center fragment read from LDS
     // feeds FB MFMAs, and a tile's copies are hidden behind FB times the work
-    int64_t frame[FB], frow[FB];
+    int64_t frame[FB], frow[FB], fpos[FB];
     bool fvalid[FB];
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
-        frame[b] = (((int64_t)blockIdx.x * NW + wave) * FB + b) * 32 + (lane & 31);
-        fvalid[b] = frame[b] < n;
+        fpos[b] = (((int64_t)blockIdx.x * NW + wave) * FB + b) * 32 + (lane & 31);
+        fvalid[b] = fpos[b] < nf;
+        if constexpr (CAND)
+            frame[b] = fvalid[b] ? rl.list[fpos[b]] : 0;
+        else
+            frame[b] = fpos[b];
         frow[b] = (fvalid[b] ? frame[b] : 0) * (int64_t)d;  // always a valid row
     }
     const int kh = (lane >> 5) * 8;  // dim offset of this half-wave inside a K-step
@@ -453,6 +506,26 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         na[b] = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
         sa[b] = sqrtf(na[b]) * (1.0f + 1e-6f);  // |a| rounded up
     }
+    // candidate mode: center j is recorded when its lower bound reaches the
+    // frame's upper bound, (na - CE sa^2 + lb_j)(1 - g) <= upper, as the
+    // threshold lb_j <= T on the epilogue's FP32 lb_j = q_j - 2 p - sa r_j,
+    // raised by 8 u (sa + max|b|)^2 so that rounding only ever adds
+    // candidates; no threshold (NaN upper): no candidate, the frame goes to
+    // the full loop
+    float thr[FB];
+#pragma unroll
+    for (int b = 0; b < FB; ++b) {
+        thr[b] = -INFINITY;
+        if constexpr (CAND) {
+            const float up = fvalid[b] ? rl.upper[fpos[b]] : __uint_as_float(0x7fc00000u);
+            if (up == up) {
+                const double sb = (double)sa[b] + (double)*cp.bmax;
+                const double T = (double)up / (1.0 - (double)gref) - ((double)na[b] - (double)CE * (double)sa[b] * (double)sa[b]) +
+                                 8.0 * 5.9604644775390625e-8 * sb * sb;
+                thr[b] = f32_round_up(T);
+            }
+        }
+    }
 
     // per frame block: running best d~ (tmin at element code icode = 16 t + e),
     // the lower bound of that candidate (lbt) and the two smallest lower bounds
@@ -497,6 +570,30 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     };
     auto epilogue = [&](const AccFB &acc, const uint8_t *img, int t) __attribute__((always_inline)) {
         const float *cst = reinterpret_cast<const float *>(img + MAT);
+        if constexpr (CAND) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int c0 = 8 * g + 4 * (lane >> 5);
+                const float4 qv = *reinterpret_cast<const float4 *>(&cst[32 + c0]);
+                const float4 rv = *reinterpret_cast<const float4 *>(&cst[64 + c0]);
+                const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+                const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+                for (int b = 0; b < FB; ++b)
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const float lb = fmaf(-sa[b], ra[h], fmaf(-2.0f, acc.v[b][4 * g + h], qa[h]));
+                        if (lb <= thr[b]) {
+                            const int j = t * 32 + c0 + h;
+                            if (j < k) {
+                                const int c = atomicAdd(&rl.ccnt[fpos[b]], 1);
+                                if (c < CAND_MAX) rl.cand[fpos[b] * CAND_MAX + c] = j;
+                            }
+                        }
+                    }
+            }
+            return;
+        }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c0 = 8 * g + 4 * (lane >> 5);
@@ -612,6 +709,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // per frame block: merge the two half-waves that hold the same frame, then
     // decide (same rule as decide_frame, written out here: the streaming
     // kernel's register allocation is sensitive to the call form)
+    if constexpr (CAND) return;
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
         float tm = tmin[b], lt = lbt[b], l1 = lb1[b], l2 = lb2[b];
@@ -633,8 +731,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         const int64_t fr = frame[b];
         if (lane < 32 && fvalid[b] && imin >= k) {
             labels[fr] = 0;  // non-finite distances: leave it to the exact loop
-            const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-            list[pos] = fr;
+            list_frame(rl, fr, __uint_as_float(0x7fc00000u));
         } else if (lane < 32 && fvalid[b]) {
             const double lbo = (lt == l1) ? (double)l2 : (double)l1;
             const double bnb = (double)cp.bn[imin];
@@ -643,10 +740,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             const double lower =
                 ((double)na[b] - (double)CE * (double)sa[b] * (double)sa[b] + lbo) * (1.0 - (double)gref);
             labels[fr] = imin;
-            if (!(lower > upper)) {
-                const unsigned long long pos = atomicAdd((unsigned long long *)list_count, 1ull);
-                list[pos] = fr;
-            }
+            if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
         }
     }
 }
@@ -666,9 +760,7 @@ __host__ __device__ constexpr int res_max_tiles(int d_pad) { return RES_LDS / ti
 template <int KSTEPS, int NW>
 __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restrict__ X, int64_t n, int d,
                                                            CenterPack cp, int k, int k_pad, float CE, float gref,
-                                                           int32_t *__restrict__ labels,
-                                                           int64_t *__restrict__ list,
-                                                           int64_t *__restrict__ list_count,
+                                                           int32_t *__restrict__ labels, RecheckLists rl,
                                                            const int32_t *__restrict__ done) {
     if (done && *done) return;
     constexpr int DP = KSTEPS * 16;
@@ -785,8 +877,7 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
             }
         }
         const int imin = icode == 0x7fffffff ? 0x7fffffff : center_of_code(icode, lane);
-        decide_frame_loose(lane, frame, fvalid, tmin, t2, imin, k, na, sa, CE, gref, bmax, cp.bn, labels, list,
-                           list_count);
+        decide_frame_loose(lane, frame, fvalid, tmin, t2, imin, k, na, sa, CE, gref, bmax, cp.bn, labels, rl);
     }
 }
 
@@ -822,18 +913,28 @@ static int pick_ksteps(int d) {
 
 struct AssignWs {
     CenterPack cp;
-    int64_t *list;
-    int64_t *count;
+    RecheckLists rl;
     size_t bytes;
 };
+
+// candidate slots: a quarter of the frames (listed frames beyond them take
+// the full loop)
+static int64_t cand_slots(int64_t n) { return n / 4 + 64; }
 
 static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     AssignWs w{};
     Carver cv(ws);
-    w.count = cv.take<int64_t>(2);  // list length, then the bits of max |b_j| (both zeroed per call)
-    w.list = cv.take<int64_t>(n > 0 ? (size_t)n : 1);
-    w.cp.bmax = w.count ? reinterpret_cast<float *>(w.count + 1) : nullptr;
+    const size_t nl = n > 0 ? (size_t)n : 1;
+    // list length, bits of max |b_j|, list2 length (zeroed per call)
+    w.rl.count = cv.take<int64_t>(4);
+    w.rl.list = cv.take<int64_t>(nl);
+    w.cp.bmax = w.rl.count ? reinterpret_cast<float *>(w.rl.count + 1) : nullptr;
     if (mode == EMMA_ASSIGN_BF16X3_RECHECK) {
+        w.rl.slots = cand_slots(n);
+        w.rl.upper = cv.take<float>(nl);
+        w.rl.ccnt = cv.take<int32_t>((size_t)w.rl.slots);
+        w.rl.cand = cv.take<int32_t>((size_t)w.rl.slots * CAND_MAX);
+        w.rl.list2 = cv.take<int64_t>(nl);
         const int ks = pick_ksteps(d);
         const int d_pad = (ks > 0 ? ks : 1) * 16;
         const int k_pad = (int)cdiv(k, 32) * 32;
@@ -843,6 +944,70 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     }
     w.bytes = cv.bytes();
     return w;
+}
+
+// Pass 3 of the recheck: the reference loop over the recorded candidates.
+// Eight consecutive threads serve one list position (thread c: candidate
+// c); the smallest distance wins, the smallest center index among equal
+// ones (the reference's first minimum: the candidates contain every center
+// the loop could choose, see RecheckLists).  Positions without usable
+// candidates go to list2.  VEC: rows read as 16-byte vectors (d % 4 == 0,
+// aligned X and C).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_assign_cand_exact(const float *__restrict__ X, int d,
+                                                           const float *__restrict__ C, int32_t *__restrict__ labels,
+                                                           RecheckLists rl, const int32_t *__restrict__ done) {
+    if (done && *done) return;
+    const int64_t total = *rl.count * CAND_MAX;  // a multiple of 8: whole groups enter each iteration
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int64_t pos = idx / CAND_MAX;
+        const int c = (int)(idx % CAND_MAX);
+        const int64_t fr = rl.list[pos];
+        int nc = -1;  // -1: the full loop
+        if (pos < rl.slots) {
+            const int cc = rl.ccnt[pos];
+            if (cc >= 1 && cc <= CAND_MAX) nc = cc;
+        }
+        float dist = INFINITY;
+        int j = 0x7fffffff;
+        if (c < nc) {
+            j = rl.cand[pos * CAND_MAX + c];
+            const float *x = X + fr * (int64_t)d;
+            const float *cr = C + (int64_t)j * d;
+            float acc = 0.0f;
+            if (VEC) {
+                for (int i = 0; i < d; i += 4) {
+                    const float4 xv = *reinterpret_cast<const float4 *>(x + i);
+                    const float4 cv = *reinterpret_cast<const float4 *>(cr + i);
+                    acc = sq_step(acc, xv.x, cv.x);
+                    acc = sq_step(acc, xv.y, cv.y);
+                    acc = sq_step(acc, xv.z, cv.z);
+                    acc = sq_step(acc, xv.w, cv.w);
+                }
+            } else {
+                for (int i = 0; i < d; ++i) acc = sq_step(acc, x[i], cr[i]);
+            }
+            dist = acc;
+        }
+#pragma unroll
+        for (int off = 1; off < CAND_MAX; off <<= 1) {
+            const float od = __shfl_xor(dist, off, 64);
+            const int oj = __shfl_xor(j, off, 64);
+            if (od < dist || (od == dist && oj < j)) {
+                dist = od;
+                j = oj;
+            }
+        }
+        if (c == 0) {
+            if (nc < 0) {
+                const unsigned long long p2 = atomicAdd((unsigned long long *)(rl.count + 2), 1ull);
+                rl.list2[p2] = fr;
+            } else {
+                labels[fr] = j;
+            }
+        }
+    }
 }
 
 // Tile variants of the MFMA kernel for each K-step count: every variant fits
@@ -897,7 +1062,7 @@ static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k,
                      int32_t *labels, const int32_t *done, hipStream_t s) {
     const int64_t blocks = cdiv(n, 32 * NW * FB);
     k_assign_mfma<KS, NW, MINB, FB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
-                                                                         w.list, w.count, done);
+                                                                         w.rl, done);
 }
 
 static int device_cus() {
@@ -920,7 +1085,7 @@ static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, 
             const int64_t cap = device_cus();
             if (blocks > cap) blocks = cap;
             k_assign_res<KS, NW><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
-                                                                    w.list, w.count, done);
+                                                                    w.rl, done);
             return;
         }
     }
@@ -949,6 +1114,17 @@ static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, 
     }
 }
 
+// pass 2 of the recheck: the MFMA kernel in candidate mode over the listed
+// frames (grid for every candidate slot; blocks past the list exit)
+template <int KS>
+static void launch_cand_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
+                           float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
+    constexpr int NW = 4, MINB = KS <= 19 ? 2 : 1;
+    const int64_t blocks = cdiv(w.rl.slots, 32 * NW);
+    k_assign_mfma<KS, NW, MINB, 1, true><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref,
+                                                                            labels, w.rl, done);
+}
+
 }  // namespace asg
 
 // shared with kmeans.hip (Lloyd)
@@ -970,7 +1146,9 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
     const int k_pad = (int)cdiv(k, 32) * 32;
     const float CE = (float)assign_error_coeff(d, d_pad);
     const float gref = (float)gamma_ref(d);
-    k_zero_i64n<<<1, 64, 0, s>>>(w.count, 2);
+    k_zero_i64n<<<1, 64, 0, s>>>(w.rl.count, 4);
+    k_zero_i32n<<<(unsigned)(cdiv(w.rl.slots, 256) < 1024 ? cdiv(w.rl.slots, 256) : 1024), 256, 0, s>>>(w.rl.ccnt,
+                                                                                                   w.rl.slots);
     k_center_mean<<<(unsigned)cdiv(d_pad, 64), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
     k_center_pack<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, CE, w.cp);
     EMMA_TRY(check_launch("assign center pack"));
@@ -985,12 +1163,32 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
         case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
     }
     EMMA_TRY(check_launch("assign mfma"));
-    // recheck: the exact loop over every center for the listed frames; the
-    // grid covers the device, the kernel reads the list length itself
+    // recheck of the listed frames: candidates by the MFMA kernel (pass 2),
+    // the reference loop over them (pass 3), the full loop over every center
+    // for the rest; the grids cover the device, the kernels read the list
+    // lengths themselves
+    switch (ks) {
+        case 2: launch_cand_ks<2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 4: launch_cand_ks<4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 8: launch_cand_ks<8>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 12: launch_cand_ks<12>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 16: launch_cand_ks<16>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 19: launch_cand_ks<19>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 24: launch_cand_ks<24>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 32: launch_cand_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+    }
+    EMMA_TRY(check_launch("assign candidates"));
+    const int64_t eb = cdiv(n * CAND_MAX, 256) < 4096 ? cdiv(n * CAND_MAX, 256) : 4096;
+    const bool vec = d % 4 == 0 && (((uintptr_t)X | (uintptr_t)C) & 15) == 0;
+    if (vec)
+        k_assign_cand_exact<true><<<(unsigned)eb, 256, 0, s>>>(X, d, C, labels, w.rl, done);
+    else
+        k_assign_cand_exact<false><<<(unsigned)eb, 256, 0, s>>>(X, d, C, labels, w.rl, done);
+    EMMA_TRY(check_launch("assign candidate recheck"));
     const int64_t rb = cdiv(n, PF) < 2048 ? cdiv(n, PF) : 2048;
-    k_assign_plain<<<(unsigned)rb, 256, 0, s>>>(X, n, d, C, k, labels, w.list, w.count, done);
+    k_assign_plain<<<(unsigned)rb, 256, 0, s>>>(X, n, d, C, k, labels, w.rl.list2, w.rl.count + 2, done);
     EMMA_TRY(check_launch("assign recheck"));
-    if (n_rechecked) k_copy_i64<<<1, 1, 0, s>>>(n_rechecked, w.count);
+    if (n_rechecked) k_copy_i64<<<1, 1, 0, s>>>(n_rechecked, w.rl.count);
     return check_launch("assign copy count");
 }
 
@@ -1095,14 +1293,17 @@ extern "C" int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const f
     if (mode == EMMA_ASSIGN_FP32_PLAIN)  // the plain loop needs no workspace
         return assign_run(X, n, d, C, k, labels, mode, n_rechecked, nullptr, nullptr, s);
     // the center images and constants take a fixed part of the scratch; the
-    // recheck list (8 bytes per frame) the rest, so the frames run in chunks
+    // recheck arrays (about 29 bytes per frame) the rest, so the frames run
+    // in chunks
     const size_t fixed = assign_workspace_bytes(0, d, k, mode);
     constexpr size_t cap = EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES;
-    const int64_t chunk_max = fixed + 512 < cap ? (int64_t)((cap - fixed - 512) / 8) : 0;
+    // per frame: list (8 bytes), its upper bound (4), list2 (8), a quarter
+    // of a candidate slot (9), with room for the 256-byte alignments
+    const int64_t chunk_max = fixed + 4096 < cap ? (int64_t)((cap - fixed - 4096) / 32) : 0;
     EMMA_REQUIRE(chunk_max >= EMMA_ASSIGN_NOALLOC_MIN_CHUNK,
                  "emma_assign_noalloc: k=%d centers of d=%d take %zu of the %zu scratch bytes; at most %lld bytes "
                  "leave room for %d frames per chunk (use emma_assign with a workspace)",
-                 k, d, fixed, cap, (long long)(cap - 512 - 8 * (size_t)EMMA_ASSIGN_NOALLOC_MIN_CHUNK),
+                 k, d, fixed, cap, (long long)(cap - 4096 - 32 * (size_t)EMMA_ASSIGN_NOALLOC_MIN_CHUNK),
                  EMMA_ASSIGN_NOALLOC_MIN_CHUNK);
     const int64_t chunk = chunk_max < ((int64_t)1 << 22) ? chunk_max : ((int64_t)1 << 22);
     EMMA_REQUIRE(assign_workspace_bytes(chunk, d, k, mode) <= cap, "emma_assign_noalloc: chunk of %lld frames "
@@ -1115,7 +1316,7 @@ extern "C" int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const f
     for (int64_t f0 = 0; f0 < n; f0 += chunk) {
         const int64_t m = n - f0 < chunk ? n - f0 : chunk;
         EMMA_TRY(assign_run(X + f0 * (int64_t)d, m, d, C, k, labels + f0, mode, nullptr, ws, nullptr, s));
-        if (n_rechecked) k_add_i64<<<1, 1, 0, s>>>(n_rechecked, w.count);
+        if (n_rechecked) k_add_i64<<<1, 1, 0, s>>>(n_rechecked, w.rl.count);
     }
     return check_launch("emma_assign_noalloc");
 }

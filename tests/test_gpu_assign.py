@@ -135,3 +135,21 @@ def test_every_tile_variant_bit_exact(d, k):
             np.testing.assert_array_equal(lab, ref, err_msg="variant %d" % v)
     finally:
         _native.check(L.emma_assign_select(d, k, 0), "emma_assign_select")
+
+
+@pytest.mark.parametrize("reps", [2, 8, 9, 12])
+def test_recheck_candidate_paths(reps):
+    """Every center repeated `reps` times: each frame ties between `reps`
+    centers.  Up to 8 candidates the tie is settled by the reference loop over
+    the recorded candidates, above 8 by the full loop; frames listed beyond a
+    quarter of the frames (the candidate slots) always take the full loop.
+    The first minimum wins in all of them."""
+    rng = np.random.default_rng(reps)
+    base = rng.standard_normal((6, 24)).astype(np.float32)
+    C = np.repeat(base, reps, axis=0)
+    X = (base[rng.integers(0, 6, 3000)] + 0.3 * rng.standard_normal((3000, 24))).astype(np.float32)
+    ref = on.assign(X, C, nthreads=4)
+    lab, nre = _run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)
+    np.testing.assert_array_equal(lab, ref)
+    assert nre == 3000
+    assert np.all(ref % reps == 0)
