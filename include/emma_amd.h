@@ -157,8 +157,8 @@ int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k,
  * Limit: the k centers' tile images, ceil(k/32) * (128 * (d_pad + 8) + 384)
  * bytes with d_pad = d rounded up to the kernel's K-step width (32, 64, 128,
  * 192, 256, 304, 384 or 512), must leave room for
- * EMMA_ASSIGN_NOALLOC_MIN_CHUNK frames (32 bytes each): k <= 14,900 at
- * d = 512, k <= 24,800 at d = 300; larger sets return EMMA_ERR_INVALID_ARG
+ * EMMA_ASSIGN_NOALLOC_MIN_CHUNK frames (64 bytes each): k <= 13,800 at
+ * d = 512, k <= 23,000 at d = 300; larger sets return EMMA_ERR_INVALID_ARG
  * with a message.  Every
  * call on one GPU shares that GPU's array: calls that overlap on one GPU
  * (different streams, threads) race and must be serialized by the caller.

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_center_mean(const float *__restrict__ C
 
 // one wavefront per (padded) center
 __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C, int k, int d, int d_pad,
-                                                     int k_pad, float CE, CenterPack cp) {
+                                                     int k_pad, float2 CE, CenterPack cp) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + wave;
     if (j >= k_pad) return;
@@ -232,8 +232,8 @@ __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C
             }
             bn = (float)(sqrt(n2s) * (1.0 + 1e-6));  // rounded up
             nb = acc;
-            q = acc - CE * bn * bn;
-            r = 2.0f * CE * bn;
+            q = acc - CE.x * bn * bn;
+            r = 2.0f * CE.y * bn;
         } else {
             nb = INFINITY;
             bn = 0.0f;
@@ -343,7 +343,7 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 // instead of 4: the resident kernel, whose time is set by its vector ALU
 // work, uses it (measured faster there; in the streaming kernel it is not).
 __device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fvalid, float tm, float t2, int imin,
-                                                   int k, float na, float sa, float CE, float gref, float bmax,
+                                                   int k, float na, float sa, float2 CE, float gref, float bmax,
                                                    const float *__restrict__ bn, int32_t *__restrict__ labels,
                                                    const RecheckLists &rl) {
     {
@@ -362,8 +362,8 @@ __device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fv
         list_frame(rl, fr, __uint_as_float(0x7fc00000u));
     } else if (lane < 32 && fvalid) {
         const double bnb = (double)bn[imin];
-        const double E1 = (double)CE * ((double)sa + bnb) * ((double)sa + bnb);
-        const double Em = (double)CE * ((double)sa + (double)bmax) * ((double)sa + (double)bmax);
+        const double E1 = (double)CE.x * ((double)sa * sa + bnb * bnb) + 2.0 * (double)CE.y * (double)sa * bnb;
+        const double Em = (double)CE.x * ((double)sa * sa + (double)bmax * bmax) + 2.0 * (double)CE.y * (double)sa * bmax;
         const double upper = ((double)na + (double)tm + E1) * (1.0 + (double)gref);
         const double lower = ((double)na + (double)t2 - Em) * (1.0 - (double)gref);
         labels[fr] = imin;
@@ -384,7 +384,7 @@ __device__ __forceinline__ int center_of_code(int code, int l) {
 // tracking the minimum.
 template <int KSTEPS, int NW, int MINB, int FB, bool CAND = false>
 __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
-                                                            CenterPack cp, int k, int k_pad, float CE,
+                                                            CenterPack cp, int k, int k_pad, float2 CE,
                                                             float gref, int32_t *__restrict__ labels,
                                                             RecheckLists rl, const int32_t *__restrict__ done) {
     if (done && *done) return;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             const float up = fvalid[b] ? rl.upper[fpos[b]] : __uint_as_float(0x7fc00000u);
             if (up == up) {
                 const double sb = (double)sa[b] + (double)*cp.bmax;
-                const double T = (double)up / (1.0 - (double)gref) - ((double)na[b] - (double)CE * (double)sa[b] * (double)sa[b]) +
+                const double T = (double)up / (1.0 - (double)gref) - ((double)na[b] - (double)CE.x * (double)sa[b] * (double)sa[b]) +
                                  8.0 * 5.9604644775390625e-8 * sb * sb;
                 thr[b] = f32_round_up(T);
             }
@@ -735,10 +735,10 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         } else if (lane < 32 && fvalid[b]) {
             const double lbo = (lt == l1) ? (double)l2 : (double)l1;
             const double bnb = (double)cp.bn[imin];
-            const double E1 = (double)CE * ((double)sa[b] + bnb) * ((double)sa[b] + bnb);
+            const double E1 = (double)CE.x * ((double)sa[b] * sa[b] + bnb * bnb) + 2.0 * (double)CE.y * (double)sa[b] * bnb;
             const double upper = ((double)na[b] + (double)tm + E1) * (1.0 + (double)gref);
             const double lower =
-                ((double)na[b] - (double)CE * (double)sa[b] * (double)sa[b] + lbo) * (1.0 - (double)gref);
+                ((double)na[b] - (double)CE.x * (double)sa[b] * (double)sa[b] + lbo) * (1.0 - (double)gref);
             labels[fr] = imin;
             if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
         }
@@ -759,7 +759,7 @@ __host__ __device__ constexpr int res_max_tiles(int d_pad) { return RES_LDS / ti
 
 template <int KSTEPS, int NW>
 __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restrict__ X, int64_t n, int d,
-                                                           CenterPack cp, int k, int k_pad, float CE, float gref,
+                                                           CenterPack cp, int k, int k_pad, float2 CE, float gref,
                                                            int32_t *__restrict__ labels, RecheckLists rl,
                                                            const int32_t *__restrict__ done) {
     if (done && *done) return;
@@ -881,20 +881,27 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
     }
 }
 
-// Coefficient CE of the per-candidate bound E_j = CE (|a| + |b_j|)^2 for
-// |d~_j - d_j|, with u = 2^-24, gamma_n = n u / (1 - n u):
+// Coefficients (c1, c2) = CE of the per-candidate bound
+//   E_j = c1 (|a|^2 + |b_j|^2) + 2 c2 |a| |b_j|   >=   |d~_j - d_j|,
+// with u = 2^-24, gamma_n = n u / (1 - n u):
 //   shift rounding of a and b:            2.1 u (|a|+|b|)^2
+//   final combination and fmaf rounding:  2.1 u (|a|+|b|)^2
 //   FP32 norms |a|^2, |b|^2:              gamma_d (|a|^2 + |b|^2)
 //   omitted bf16 split terms:             3.1 * 2^-16 * 2 sum|a_i b_i|
-//   FP32 MFMA accumulation of 3 d_pad terms: 1.02 gamma_{3 d_pad} * 2 sum|a_i b_i|
-//   final combination and fmaf rounding:  2.1 u (|a|+|b|)^2
-// using 2 sum|a_i b_i| <= (|a|+|b|)^2.  A 25 % margin covers the rounding of
-// the FP32 bound arithmetic itself.
-static double assign_error_coeff(int d, int d_pad) {
+//   FP32 MFMA accumulation of 3 d_pad terms (each addition rounded to
+//   nearest, as the device probe of the matrix core shows):
+//                                         1.02 gamma_{3 d_pad} * 2 sum|a_i b_i|
+// using (|a|+|b|)^2 = |a|^2 + |b|^2 + 2|a||b| and sum|a_i b_i| <= |a||b|, so
+// the product terms are charged on 2|a||b| only.  A 25 % margin covers the
+// rounding of the FP32 bound arithmetic itself.
+static float2 assign_error_coeffs(int d, int d_pad) {
     const double u = ldexp(1.0, -24);
     auto gam = [&](double m) { return m * u / (1.0 - m * u); };
-    const double c = 4.2 * u + gam(d) + 3.1 * ldexp(1.0, -16) + 1.02 * gam(3.0 * d_pad);
-    return 1.25 * c;
+    const double c0 = 4.2 * u;
+    float2 ce;
+    ce.x = (float)(1.25 * (c0 + gam(d)));
+    ce.y = (float)(1.25 * (c0 + 3.1 * ldexp(1.0, -16) + 1.02 * gam(3.0 * d_pad)));
+    return ce;
 }
 
 static double gamma_ref(int d) {
@@ -917,9 +924,10 @@ struct AssignWs {
     size_t bytes;
 };
 
-// candidate slots: a quarter of the frames (listed frames beyond them take
-// the full loop)
-static int64_t cand_slots(int64_t n) { return n / 4 + 64; }
+// candidate slots: one per frame that can be listed (near-tie-heavy center
+// sets, such as the cluster means after a Lloyd update with many centers per
+// mode of the data, list about half of the frames)
+static int64_t cand_slots(int64_t n) { return n > 0 ? n : 1; }
 
 static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     AssignWs w{};
@@ -947,7 +955,7 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
 }
 
 // Pass 3 of the recheck: the reference loop over the recorded candidates.
-// Eight consecutive threads serve one list position (thread c: candidate
+// CAND_MAX consecutive threads serve one list position (thread c: candidate
 // c); the smallest distance wins, the smallest center index among equal
 // ones (the reference's first minimum: the candidates contain every center
 // the loop could choose, see RecheckLists).  Positions without usable
@@ -958,7 +966,7 @@ __global__ __launch_bounds__(256) void k_assign_cand_exact(const float *__restri
                                                            const float *__restrict__ C, int32_t *__restrict__ labels,
                                                            RecheckLists rl, const int32_t *__restrict__ done) {
     if (done && *done) return;
-    const int64_t total = *rl.count * CAND_MAX;  // a multiple of 8: whole groups enter each iteration
+    const int64_t total = *rl.count * CAND_MAX;  // whole groups enter each iteration (the stride is a multiple)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
         const int64_t pos = idx / CAND_MAX;
@@ -1058,7 +1066,7 @@ static int selected_variant(int ks, int k_pad) {
 }
 
 template <int KS, int NW, int MINB, int FB = 1>
-static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE, float gref,
+static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE, float gref,
                      int32_t *labels, const int32_t *done, hipStream_t s) {
     const int64_t blocks = cdiv(n, 32 * NW * FB);
     k_assign_mfma<KS, NW, MINB, FB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
@@ -1074,7 +1082,7 @@ static int device_cus() {
 }
 
 template <int KS>
-static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
+static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     const int v = selected_variant(KS, k_pad);
     if constexpr (KS <= 8) {
@@ -1117,7 +1125,7 @@ static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, 
 // pass 2 of the recheck: the MFMA kernel in candidate mode over the listed
 // frames (grid for every candidate slot; blocks past the list exit)
 template <int KS>
-static void launch_cand_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float CE,
+static void launch_cand_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     constexpr int NW = 4, MINB = KS <= 19 ? 2 : 1;
     const int64_t blocks = cdiv(w.rl.slots, 32 * NW);
@@ -1144,7 +1152,7 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
     EMMA_REQUIRE(ks > 0, "emma_assign: d=%d exceeds the MFMA path limit (512)", d);
     const int d_pad = ks * 16;
     const int k_pad = (int)cdiv(k, 32) * 32;
-    const float CE = (float)assign_error_coeff(d, d_pad);
+    const float2 CE = assign_error_coeffs(d, d_pad);
     const float gref = (float)gamma_ref(d);
     k_zero_i64n<<<1, 64, 0, s>>>(w.rl.count, 4);
     k_zero_i32n<<<(unsigned)(cdiv(w.rl.slots, 256) < 1024 ? cdiv(w.rl.slots, 256) : 1024), 256, 0, s>>>(w.rl.ccnt,
@@ -1293,17 +1301,17 @@ extern "C" int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const f
     if (mode == EMMA_ASSIGN_FP32_PLAIN)  // the plain loop needs no workspace
         return assign_run(X, n, d, C, k, labels, mode, n_rechecked, nullptr, nullptr, s);
     // the center images and constants take a fixed part of the scratch; the
-    // recheck arrays (about 29 bytes per frame) the rest, so the frames run
+    // recheck arrays (56 bytes per frame) the rest, so the frames run
     // in chunks
     const size_t fixed = assign_workspace_bytes(0, d, k, mode);
     constexpr size_t cap = EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES;
-    // per frame: list (8 bytes), its upper bound (4), list2 (8), a quarter
-    // of a candidate slot (9), with room for the 256-byte alignments
-    const int64_t chunk_max = fixed + 4096 < cap ? (int64_t)((cap - fixed - 4096) / 32) : 0;
+    // per frame: list (8 bytes), its upper bound (4), list2 (8), a candidate
+    // slot (36), with room for the 256-byte alignments
+    const int64_t chunk_max = fixed + 8192 < cap ? (int64_t)((cap - fixed - 8192) / 64) : 0;
     EMMA_REQUIRE(chunk_max >= EMMA_ASSIGN_NOALLOC_MIN_CHUNK,
                  "emma_assign_noalloc: k=%d centers of d=%d take %zu of the %zu scratch bytes; at most %lld bytes "
                  "leave room for %d frames per chunk (use emma_assign with a workspace)",
-                 k, d, fixed, cap, (long long)(cap - 4096 - 32 * (size_t)EMMA_ASSIGN_NOALLOC_MIN_CHUNK),
+                 k, d, fixed, cap, (long long)(cap - 8192 - 64 * (size_t)EMMA_ASSIGN_NOALLOC_MIN_CHUNK),
                  EMMA_ASSIGN_NOALLOC_MIN_CHUNK);
     const int64_t chunk = chunk_max < ((int64_t)1 << 22) ? chunk_max : ((int64_t)1 << 22);
     EMMA_REQUIRE(assign_workspace_bytes(chunk, d, k, mode) <= cap, "emma_assign_noalloc: chunk of %lld frames "

@@ -140,10 +140,9 @@ def test_every_tile_variant_bit_exact(d, k):
 @pytest.mark.parametrize("reps", [2, 8, 9, 12])
 def test_recheck_candidate_paths(reps):
     """Every center repeated `reps` times: each frame ties between `reps`
-    centers.  Up to 8 candidates the tie is settled by the reference loop over
-    the recorded candidates, above 8 by the full loop; frames listed beyond a
-    quarter of the frames (the candidate slots) always take the full loop.
-    The first minimum wins in all of them."""
+    centers (every frame is listed).  Up to 8 candidates the tie is settled by
+    the reference loop over the recorded candidates, above 8 by the full loop
+    over every center.  The first minimum wins in both."""
     rng = np.random.default_rng(reps)
     base = rng.standard_normal((6, 24)).astype(np.float32)
     C = np.repeat(base, reps, axis=0)
