@@ -235,9 +235,11 @@ __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C
             q = acc - CE.x * bn * bn;
             r = 2.0f * CE.y * bn;
         } else {
-            nb = INFINITY;
+            // padding: a finite value beyond every real d~ (the epilogue's
+            // index bits must not turn it into a NaN); never a candidate
+            nb = 3.0e38f;
             bn = 0.0f;
-            q = INFINITY;
+            q = 3.0e38f;
             r = 0.0f;
         }
         cp.bn[j] = bn;
@@ -332,50 +334,67 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
     }
 }
 
+// Running state of the epilogue per frame block and lane: the two smallest
+// values v_j = d~_j - |a|^2 = |b_j|^2 - 2 a.b_j seen so far (m1 <= m2 as a
+// multiset), each with the element e (0..15) of its 32 x 32 tile in the 4 low
+// mantissa bits, and the tile of m1.  Per element: one fma for v, one
+// and-or for the index, med3 and min for the pair (m2 = med3(m1, m2, v) is
+// the new second smallest when m1 <= m2).
+__device__ __forceinline__ float pack_elem(float v, int e) {
+    return __uint_as_float((__float_as_uint(v) & 0xFFFFFFF0u) | (uint32_t)e);
+}
+// min(m1, vp) as the median of (m1, vp, -inf): v_min_f32 would first
+// canonicalize the bit-built vp (one more instruction per element), and the
+// compiler folds a median with a literal -inf back into that min, so the -inf
+// comes from opaque_neg_inf() (an SGPR set once per kernel).
+__device__ __forceinline__ float opaque_neg_inf() {
+    float v = -INFINITY;
+    asm volatile("" : "+s"(v));
+    return v;
+}
+__device__ __forceinline__ void track2(float &m1, float &m2, float vp, float ninf) {
+    m2 = __builtin_amdgcn_fmed3f(m1, m2, vp);
+    m1 = __builtin_amdgcn_fmed3f(m1, vp, ninf);
+}
+
 // Label of one frame from the running state of its two half-waves (lane and
 // lane ^ 32 hold the same frame, different centers), and whether it needs
-// the exact recheck (lanes < 32 write).  The rule of the streaming kernel,
-// but from the two smallest d~ (tmin of the winner, t2 of the runner-up) and
-// one bound for every other candidate, E_max = CE (|a| + max_j
-// |b_j|)^2 >= E_j: for j != best, d_j >= d~_j - E_j >= t2 - E_max, and d_best
-// <= tmin + E_best.  Looser than the per-candidate form (about a tenth more
-// frames go to the recheck) but the epilogue tracks 2 values per element
-// instead of 4: the resident kernel, whose time is set by its vector ALU
-// work, uses it (measured faster there; in the streaming kernel it is not).
-__device__ __forceinline__ void decide_frame_loose(int lane, int64_t fr, bool fvalid, float tm, float t2, int imin,
-                                                   int k, float na, float sa, float2 CE, float gref, float bmax,
-                                                   const float *__restrict__ bn, int32_t *__restrict__ labels,
-                                                   const RecheckLists &rl) {
+// the recheck (lanes < 32 write).  With E_max = c1 (|a|^2 + max|b|^2) +
+// 2 c2 |a| max|b| >= E_j for every center: for j != best,
+// d_j >= d~_j - E_j >= |a|^2 + m2 - E_max, and d_best <= |a|^2 + m1 + E_best
+// (both through the reference loop's relative rounding g).  A listed frame
+// carries the upper bound of its best candidate to the candidate pass.
+__device__ __forceinline__ void decide_frame(int lane, int64_t fr, bool fvalid, float m1, float m2, int mt, int k,
+                                             float na, float sa, float2 CE, float gref, float bmax,
+                                             const float *__restrict__ bn, int32_t *__restrict__ labels,
+                                             const RecheckLists &rl) {
+    int half = lane >> 5;
     {
-        const float ot = __shfl_xor(tm, 32, 64);
-        const int oi = __shfl_xor(imin, 32, 64);
-        const float o2 = __shfl_xor(t2, 32, 64);
-        // second smallest of the union of {tm, t2} and {ot, o2}
-        t2 = fminf(fminf(t2, o2), fmaxf(tm, ot));
-        if (ot < tm || (ot == tm && oi < imin)) {
-            tm = ot;
-            imin = oi;
+        const float o1 = __shfl_xor(m1, 32, 64);
+        const float o2 = __shfl_xor(m2, 32, 64);
+        const int ot = __shfl_xor(mt, 32, 64);
+        m2 = fminf(fminf(m2, o2), fmaxf(m1, o1));
+        if (o1 < m1 || (o1 == m1 && (ot < mt || (ot == mt && half == 1)))) {
+            m1 = o1;
+            mt = ot;
+            half ^= 1;
         }
     }
+    const int e = (int)(__float_as_uint(m1) & 15u);
+    const int imin = (mt >= 0 && m1 == m1 && m1 < 1e38f) ? mt * 32 + 8 * (e >> 2) + 4 * half + (e & 3) : 0x7fffffff;
     if (lane < 32 && fvalid && imin >= k) {
         labels[fr] = 0;  // non-finite distances: leave it to the exact loop
         list_frame(rl, fr, __uint_as_float(0x7fc00000u));
     } else if (lane < 32 && fvalid) {
+        const double a2 = (double)sa * sa;
         const double bnb = (double)bn[imin];
-        const double E1 = (double)CE.x * ((double)sa * sa + bnb * bnb) + 2.0 * (double)CE.y * (double)sa * bnb;
-        const double Em = (double)CE.x * ((double)sa * sa + (double)bmax * bmax) + 2.0 * (double)CE.y * (double)sa * bmax;
-        const double upper = ((double)na + (double)tm + E1) * (1.0 + (double)gref);
-        const double lower = ((double)na + (double)t2 - Em) * (1.0 - (double)gref);
+        const double E1 = (double)CE.x * (a2 + bnb * bnb) + 2.0 * (double)CE.y * (double)sa * bnb;
+        const double Em = (double)CE.x * (a2 + (double)bmax * bmax) + 2.0 * (double)CE.y * (double)sa * bmax;
+        const double upper = ((double)na + (double)m1 + E1) * (1.0 + (double)gref);
+        const double lower = ((double)na + (double)m2 - Em) * (1.0 - (double)gref);
         labels[fr] = imin;
         if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
     }
-}
-
-// center index of the element code of a 32x32 output tile: tile t, element e
-// (lane l holds centers 8 (e >> 2) + 4 (l >> 5) + (e & 3))
-__device__ __forceinline__ int center_of_code(int code, int l) {
-    const int tt = code >> 4, e = code & 15;
-    return tt * 32 + 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
 }
 
 // CAND: pass 2 of the recheck (candidate mode): the frames are the listed
@@ -527,15 +546,15 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         }
     }
 
-    // per frame block: running best d~ (tmin at element code icode = 16 t + e),
-    // the lower bound of that candidate (lbt) and the two smallest lower bounds
-    // over all candidates (lb1 <= lb2, as a multiset)
-    float tmin[FB], lbt[FB], lb1[FB], lb2[FB];
-    int icode[FB];
+    // per frame block: the two smallest d~ - |a|^2 with the element index in
+    // the low bits, and the tile of the smallest (see track2)
+    float m1[FB], m2[FB];
+    int mt[FB];
+    const float ninf = opaque_neg_inf();
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
-        tmin[b] = lbt[b] = lb1[b] = lb2[b] = INFINITY;
-        icode[b] = 0x7fffffff;
+        m1[b] = m2[b] = INFINITY;
+        mt[b] = -1;
     }
 
     struct AccFB {
@@ -594,39 +613,34 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             }
             return;
         }
+        float prev[FB];
+#pragma unroll
+        for (int b = 0; b < FB; ++b) prev[b] = m1[b];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c0 = 8 * g + 4 * (lane >> 5);
             const float4 nbv = *reinterpret_cast<const float4 *>(&cst[c0]);
-            const float4 qv = *reinterpret_cast<const float4 *>(&cst[32 + c0]);
-            const float4 rv = *reinterpret_cast<const float4 *>(&cst[64 + c0]);
             const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
-            const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
-            const float ra[4] = {rv.x, rv.y, rv.z, rv.w};
 #pragma unroll
             for (int b = 0; b < FB; ++b)
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    const float p = acc.v[b][4 * g + h];
-                    const float tv = fmaf(-2.0f, p, nba[h]);
-                    const float lb = fmaf(-sa[b], ra[h], fmaf(-2.0f, p, qa[h]));
-                    const bool c1 = tv < tmin[b];  // strict: the first minimum wins
-                    tmin[b] = c1 ? tv : tmin[b];
-                    icode[b] = c1 ? t * 16 + 4 * g + h : icode[b];
-                    lbt[b] = c1 ? lb : lbt[b];
-                    lb2[b] = fminf(lb2[b], fmaxf(lb, lb1[b]));
-                    lb1[b] = fminf(lb1[b], lb);
-                }
+                for (int h = 0; h < 4; ++h)
+                    track2(m1[b], m2[b], pack_elem(fmaf(-2.0f, acc.v[b][4 * g + h], nba[h]), 4 * g + h), ninf);
         }
+#pragma unroll
+        for (int b = 0; b < FB; ++b) mt[b] = m1[b] != prev[b] ? t : mt[b];
     };
     // MFMAs of one tile interleaved with the epilogue of the previous one
+    // (about 4 VALU per element, 16 elements per frame block, spread over the
+    // K-steps)
+    constexpr int EV = (68 * FB + KSTEPS - 1) / KSTEPS;
     auto interleave = [&]() __attribute__((always_inline)) {
         __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operand reads of K-step 0
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
             if (s + 1 < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operands of K-step s + 1
             __builtin_amdgcn_sched_group_barrier(0x008, 3 * FB, 0);                 // 3 FB MFMAs of K-step s
-            __builtin_amdgcn_sched_group_barrier(0x002, 9 * FB, 0);                 // epilogue VALU
+            __builtin_amdgcn_sched_group_barrier(0x002, EV, 0);                     // epilogue VALU
         }
     };
 
@@ -706,43 +720,13 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     epilogue(acc, (ntiles & 1) ? s_img1 : s_img0, ntiles - 1);
     }
 
-    // per frame block: merge the two half-waves that hold the same frame, then
-    // decide (same rule as decide_frame, written out here: the streaming
-    // kernel's register allocation is sensitive to the call form)
+    // per frame block: merge the two half-waves that hold the same frame and decide
     if constexpr (CAND) return;
+    const float bmax = *cp.bmax;
 #pragma unroll
-    for (int b = 0; b < FB; ++b) {
-        float tm = tmin[b], lt = lbt[b], l1 = lb1[b], l2 = lb2[b];
-        int imin = icode[b] == 0x7fffffff ? 0x7fffffff : center_of_code(icode[b], lane);
-        {
-            const float ot = __shfl_xor(tm, 32, 64);
-            const int oi = __shfl_xor(imin, 32, 64);
-            const float olt = __shfl_xor(lt, 32, 64);
-            const float o1 = __shfl_xor(l1, 32, 64);
-            const float o2 = __shfl_xor(l2, 32, 64);
-            if (ot < tm || (ot == tm && oi < imin)) {
-                tm = ot;
-                imin = oi;
-                lt = olt;
-            }
-            l2 = fminf(fminf(l2, o2), fmaxf(l1, o1));
-            l1 = fminf(l1, o1);
-        }
-        const int64_t fr = frame[b];
-        if (lane < 32 && fvalid[b] && imin >= k) {
-            labels[fr] = 0;  // non-finite distances: leave it to the exact loop
-            list_frame(rl, fr, __uint_as_float(0x7fc00000u));
-        } else if (lane < 32 && fvalid[b]) {
-            const double lbo = (lt == l1) ? (double)l2 : (double)l1;
-            const double bnb = (double)cp.bn[imin];
-            const double E1 = (double)CE.x * ((double)sa[b] * sa[b] + bnb * bnb) + 2.0 * (double)CE.y * (double)sa[b] * bnb;
-            const double upper = ((double)na[b] + (double)tm + E1) * (1.0 + (double)gref);
-            const double lower =
-                ((double)na[b] - (double)CE.x * (double)sa[b] * (double)sa[b] + lbo) * (1.0 - (double)gref);
-            labels[fr] = imin;
-            if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
-        }
-    }
+    for (int b = 0; b < FB; ++b)
+        decide_frame(lane, frame[b], fvalid[b], m1[b], m2[b], mt[b], k, na[b], sa[b], CE, gref, bmax, cp.bn, labels,
+                     rl);
 }
 
 // Resident-center form for small k d (every center tile image fits in LDS):
@@ -854,30 +838,27 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
         const float sa = sqrtf(na) * (1.0f + 1e-6f);  // |a| rounded up
         if (blk + stride < nblk) load_raw(blk + stride);
 
-        // the two smallest d~ (decide_frame_loose)
-        float tmin = INFINITY, t2 = INFINITY;
-        int icode = 0x7fffffff;
+        // the two smallest d~ - |a|^2 and the tile of the smallest (track2)
+        const float ninf = opaque_neg_inf();
+        float m1 = INFINITY, m2 = INFINITY;
+        int mt = -1;
         for (int t = 0; t < ntiles; ++t) {
             const uint8_t *img = s_res + t * IMG;
             const floatx16 acc = chain(img, bhi, blo);
             const float *cst = reinterpret_cast<const float *>(img + MAT);
+            const float prev = m1;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int c0 = 8 * g + 4 * (lane >> 5);
                 const float4 nbv = *reinterpret_cast<const float4 *>(&cst[c0]);
                 const float nba[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    const float tv = fmaf(-2.0f, acc[4 * g + h], nba[h]);
-                    const bool c1 = tv < tmin;  // strict: the first minimum wins
-                    t2 = fminf(t2, fmaxf(tv, tmin));
-                    tmin = c1 ? tv : tmin;
-                    icode = c1 ? t * 16 + 4 * g + h : icode;
-                }
+                for (int h = 0; h < 4; ++h)
+                    track2(m1, m2, pack_elem(fmaf(-2.0f, acc[4 * g + h], nba[h]), 4 * g + h), ninf);
             }
+            mt = m1 != prev ? t : mt;
         }
-        const int imin = icode == 0x7fffffff ? 0x7fffffff : center_of_code(icode, lane);
-        decide_frame_loose(lane, frame, fvalid, tmin, t2, imin, k, na, sa, CE, gref, bmax, cp.bn, labels, rl);
+        decide_frame(lane, frame, fvalid, m1, m2, mt, k, na, sa, CE, gref, bmax, cp.bn, labels, rl);
     }
 }
 
@@ -886,6 +867,7 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
 // with u = 2^-24, gamma_n = n u / (1 - n u):
 //   shift rounding of a and b:            2.1 u (|a|+|b|)^2
 //   final combination and fmaf rounding:  2.1 u (|a|+|b|)^2
+//   element index in the low mantissa bits: 32 u (|a|+|b|)^2
 //   FP32 norms |a|^2, |b|^2:              gamma_d (|a|^2 + |b|^2)
 //   omitted bf16 split terms:             3.1 * 2^-16 * 2 sum|a_i b_i|
 //   FP32 MFMA accumulation of 3 d_pad terms (each addition rounded to
@@ -897,7 +879,9 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
 static float2 assign_error_coeffs(int d, int d_pad) {
     const double u = ldexp(1.0, -24);
     auto gam = [&](double m) { return m * u / (1.0 - m * u); };
-    const double c0 = 4.2 * u;
+    // + 32 u: the epilogue keeps an element index in the 4 low mantissa bits
+    // of d~ - |a|^2 (|d~ - |a|^2| <= (|a|+|b|)^2), a relative change < 2^-19
+    const double c0 = 4.2 * u + 32.0 * u;
     float2 ce;
     ce.x = (float)(1.25 * (c0 + gam(d)));
     ce.y = (float)(1.25 * (c0 + 3.1 * ldexp(1.0, -16) + 1.02 * gam(3.0 * d_pad)));
