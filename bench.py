@@ -92,6 +92,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 GPU comparison")
     ap.add_argument("--no-hbm-legs", action="store_true", help="skip the small-d assignment legs")
     ap.add_argument("--no-large-k", action="store_true", help="skip the 10^7 x 512, k = 4096 k-means leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 4096-trajectory batched leg")
     ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
     ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving leg (0: skip)")
@@ -252,13 +253,15 @@ def main():
     roofline.update({"executed_tflops": ex, "executed_frac": ex / PEAK_BF16_TFLOPS})
 
     serving = None
-    if args.serve_seconds > 0 or not args.no_hbm_legs or not args.no_large_k:
+    if args.serve_seconds > 0 or not args.no_hbm_legs or not args.no_large_k or not args.no_batched:
         del X, sub, labels
         torch.cuda.empty_cache()
     if not args.no_hbm_legs:
         kern.update(assign_hbm_legs())
     if not args.no_large_k:
         kern["large_k"] = large_k_leg(args)
+    if not args.no_batched:
+        kern["batched"] = batched_leg(args)
     if args.serve_seconds > 0:
         serving = serving_leg(args)
 
@@ -534,6 +537,91 @@ def large_k_leg(args, n=10_000_000, d=512, k=4096, n_seed=200_000, iters=5, n_pl
                                 % (n_cpu, k, cores, t_cpu, n)}
     log("large-k parity: %s" % out["parity"])
     del X, sub, lab, lab_plain, res
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_leg(args, ntraj=4096, d=128, lag=5, k=512, n_seed=1_000_000, n_plain=2_000_000, reps=3):
+    """The batched workload at full size: 4096 linear-Langevin trajectories with
+    log-uniform lengths 1k - 200k frames (about 1.5e8 frames x 128 float32,
+    about 79 GB, generated on the device), TICA lag 5 moments and the
+    assignment of every frame to k = 512 centers (kmeans++ on a seeded 1M-frame
+    subsample), run as one captured HIP graph (BatchedDiscretizer).  Reports
+    the replay time, the per-call times of the eager pass, the rooflines
+    (moments: executed bf16 MFMA rate; assignment: HBM bytes (4 d + 4) N + 4 k d
+    against 8 TB/s) and parity: graph replay == eager bit for bit, labels vs
+    the plain FP32 loop on the first n_plain frames, every TICA eigenvalue of
+    the default mode vs the FP64-tile moments of the whole data."""
+    import torch
+
+    from oracle import eig as oeig
+    from pyemma_amd import _native
+    from pyemma_amd.batch import BatchedDiscretizer
+    from pyemma_amd.synthetic import langevin_trajectories
+
+    dev = _native.device()
+    t0 = time.time()
+    X, off = langevin_trajectories(ntraj, d, lmin=1000, lmax=200000, seed=4, device=dev)
+    torch.cuda.synchronize()
+    N = int(off[-1])
+    log("batched data: %d trajectories, %d frames x %d generated in %.1f s" % (ntraj, N, d, time.time() - t0))
+    rng = np.random.default_rng(41)
+    sub = X[torch.as_tensor(np.sort(rng.choice(N, size=n_seed, replace=False)), device=dev)].contiguous()
+    C, _ = _native.kmeanspp(sub, k, 4343)
+    del sub
+    b = BatchedDiscretizer(X, off, lag, C)
+    w0, mean0, C00_0, C0t_0, lab0 = b.run_eager()  # eager pass (also tunes the tile shapes)
+    b.capture()  # the graph's outputs are filled by its replays
+    # per-call times of the eager sequence
+    off_dev = torch.as_tensor(np.asarray(off, dtype=np.int64), device=dev)
+    ms_mom = _timed(lambda: _native.moments_lagged(X, off_dev, lag, reversible=True, remove_mean=True, c0t=True,
+                                                   precision=_native.EMMA_PREC_BF16X4_ACC), reps=1)
+    nre = torch.zeros(1, dtype=torch.int64, device=dev)
+    lab = torch.empty(N, dtype=torch.int32, device=dev)
+    ms_asg = _timed(lambda: _native.assign(X, C, labels=lab, n_rechecked=nre), reps=1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.replay()
+    e1.record()
+    e1.synchronize()
+    ms_graph = e0.elapsed_time(e1) / reps
+    w1, mean1, C00_1, C0t_1, lab1 = b.results()
+    same = bool(w1 == w0 and np.array_equal(mean1, mean0) and np.array_equal(C00_1, C00_0)
+                and np.array_equal(C0t_1, C0t_0) and np.array_equal(lab1, lab0))
+    P = int(sum(max(0, int(e) - int(a) - lag) for a, e in zip(off[:-1], off[1:])))
+    ex_mom = mom_executed_flops(d, P)
+    asg_bytes = (4 * d + 4) * N + 4 * k * d
+    out = {"workload": "%d Langevin trajectories, lengths log-uniform [1e3, 2e5], %d frames x %d f32, lag %d, k=%d"
+                       % (ntraj, N, d, lag, k),
+           "frames": N, "lagged_pairs": P, "graph_replay_ms": ms_graph, "frames_per_s": N / (ms_graph * 1e-3),
+           "moments": {"ms": ms_mom, "executed_tflop": ex_mom / 1e12,
+                       "executed_frac": ex_mom / (ms_mom * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+                       "algorithmic_tflop": 6.0 * d * d * P / 1e12},
+           "assign": {"ms": ms_asg, "bytes": asg_bytes, "gbs": asg_bytes / (ms_asg * 1e-3) / 1e9,
+                      "hbm_frac": asg_bytes / (ms_asg * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                      "rechecked_frames": int(nre.item()),
+                      "executed_frac": asg_executed_flops(d, k, N) / (ms_asg * 1e-3) / 1e12 / PEAK_BF16_TFLOPS}}
+    # parity
+    lab_plain = torch.empty(n_plain, dtype=torch.int32, device=dev)
+    _native.assign(X[:n_plain], C, mode=_native.EMMA_ASSIGN_FP32_PLAIN, labels=lab_plain)
+    mism = int(np.count_nonzero(lab_plain.cpu().numpy() != lab0[:n_plain]))
+    w, _, _, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, lag, reversible=True, remove_mean=True, c0t=True,
+                                                  precision=_native.EMMA_PREC_FP64)
+    wh = w.item()
+    ev64 = oeig.generalized_eigenvalues(Mxx.cpu().numpy() / wh, Mxy.cpu().numpy() / wh)
+    ev = oeig.generalized_eigenvalues(C00_0, C0t_0)
+    big = np.abs(ev64) >= 0.1
+    out["parity"] = {"graph_replay_equals_eager": same, "label_mismatches_vs_plain_gpu": mism,
+                     "label_frames_vs_plain_gpu": n_plain,
+                     "eig_rel_err_max_vs_fp64_tiles": float(np.max(np.abs(ev - ev64) / np.abs(ev64))),
+                     "eig_rel_err_max_abs_ge_0.1_vs_fp64_tiles": float(np.max(np.abs(ev - ev64)[big] / np.abs(ev64[big])))
+                     if big.any() else None,
+                     "eig_abs_err_max_vs_fp64_tiles": float(np.max(np.abs(ev - ev64))),
+                     "eig_count": int(len(ev)), "eig_count_abs_ge_0.1": int(big.sum())}
+    log("batched: graph replay %.1f ms (%.3g frames/s); moments %.1f ms, assign %.1f ms (%.0f GB/s); parity %s"
+        % (ms_graph, out["frames_per_s"], ms_mom, ms_asg, out["assign"]["gbs"], out["parity"]))
+    del X, b, lab, lab_plain, C
     torch.cuda.empty_cache()
     return out
 
