@@ -154,11 +154,11 @@ int emma_assign(const float *X, int64_t n, int32_t d, const float *C, int32_t k,
  * scratch (center images, recheck list) is a fixed device array of
  * EMMA_ASSIGN_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds
  * until the process ends; frames are processed in chunks that fit it.
- * Limit: the k centers' tile images, ceil(k/32) * (128 * (d_pad + 8) + 384)
- * bytes with d_pad = d rounded up to the kernel's K-step width (32, 64, 128,
- * 192, 256, 304, 384 or 512), must leave room for
- * EMMA_ASSIGN_NOALLOC_MIN_CHUNK frames (64 bytes each): k <= 13,800 at
- * d = 512, k <= 23,000 at d = 300; larger sets return EMMA_ERR_INVALID_ARG
+ * Limit: the k centers' tile images, ceil(k/32) * (192 * (d_pad + 8) + 768)
+ * bytes (bf16 and fp16 images) with d_pad = d rounded up to the kernel's
+ * K-step width (32, 64, 128, 192, 256, 304, 384 or 512), must leave room for
+ * EMMA_ASSIGN_NOALLOC_MIN_CHUNK frames (64 bytes each): k <= 9,300 at
+ * d = 512, k <= 15,400 at d = 300; larger sets return EMMA_ERR_INVALID_ARG
  * with a message.  Every
  * call on one GPU shares that GPU's array: calls that overlap on one GPU
  * (different streams, threads) race and must be serialized by the caller.
@@ -169,8 +169,11 @@ int emma_assign_noalloc(const float *X, int64_t n, int32_t d, const float *C, in
                         int32_t mode, int64_t *n_rechecked, void *stream);
 
 /*
- * Tile variants of the MFMA assignment kernel (waves per workgroup and
- * workgroups per CU), for the autotuner.  emma_assign_variants(d) is the
+ * Variants of the MFMA assignment, for the autotuner: the pipeline of its
+ * first pass (a coarse fp16 pass over every frame whose undecided frames go
+ * to the fine bf16 x 3 pass, or the fine pass over every frame) and the tile
+ * shape (waves per workgroup, workgroups per CU, frame blocks per wave).
+ * emma_assign_variants(d) is the
  * number of variants at dimension d (0 if d is beyond the MFMA path);
  * emma_assign_select(d, k, v) makes variant v the one used for every later
  * call whose dimension and center count fall in the same bucket as (d, k)

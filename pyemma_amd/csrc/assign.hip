@@ -173,14 +173,19 @@ __global__ void k_add_i64(int64_t *dst, const int64_t *src) { *dst += *src; }
 //   float  nb[32], q[32], r[32]           |b|^2 (FP32 sequential), nb - CE bn^2, 2 CE bn
 // so that a tile moves from L2 to LDS with asynchronous 16-byte copies.
 struct CenterPack {
-    uint8_t *img;  // k_pad / 32 tile images of img_bytes each
-    float *bn;     // [k_pad] |b| rounded up
-    float *mu;     // [d_pad] shift
-    float *bmax;   // max_j |b_j| (zeroed before the packing, then an atomic max)
+    uint8_t *img;    // k_pad / 32 tile images of img_bytes each
+    uint8_t *img16;  // k_pad / 32 fp16 tile images of img16_bytes each (the coarse pass)
+    float *bn;       // [k_pad] |b| rounded up
+    float *mu;       // [d_pad] shift
+    float *bmax;     // max_j |b_j| (zeroed before the packing, then an atomic max)
+    float *sc16;     // [4] coarse pass: 2^t, -2^(1 - 2t), the coefficient c3 of its absolute error term
 };
 
 __host__ __device__ constexpr int tile_pitch(int d_pad) { return d_pad + 8; }
 __host__ __device__ constexpr int tile_img_bytes(int d_pad) { return 128 * tile_pitch(d_pad) + 3 * 32 * 4; }
+// fp16 image of the coarse pass: uint16 h[32][pitch] (fp16 bits of 2^t b),
+// then the same constants nb, q, r
+__host__ __device__ constexpr int tile_img16_bytes(int d_pad) { return 64 * tile_pitch(d_pad) + 3 * 32 * 4; }
 
 // mu = mean of the centers (FP64 in a fixed order, rounded to FP32):
 // 64 dims per workgroup, wave w sums centers w, w + 4, ...
@@ -250,6 +255,48 @@ __global__ __launch_bounds__(256) void k_center_pack(const float *__restrict__ C
     }
 }
 
+// Scale of the coarse pass: 2^t with t = 9 - e, bmax = m 2^e (m in [1/2, 1)),
+// clamped to |t| <= 60, so that every |2^t b_ji| < 2^10 (fp16 holds up to
+// 65504: frames with |a_i| < 64 max|b| stay finite; larger ones overflow to
+// inf and go to the fine pass).  c3 bounds the effect of fp16 underflow
+// (absolute error <= 2^-25 of a scaled component) on d~:
+// 2 * 1.01 * 2^-25 2^-t sqrt(d) (|a| + |b_j|), with a 25 % margin.
+__global__ void k_center_scale16(int d, CenterPack cp) {
+    const float bm = *cp.bmax;
+    int e = 0;
+    frexpf(bm, &e);
+    int t = bm > 0.0f && bm == bm && bm < INFINITY ? 9 - e : 0;
+    t = t > 60 ? 60 : (t < -60 ? -60 : t);
+    cp.sc16[0] = ldexpf(1.0f, t);
+    cp.sc16[1] = -ldexpf(1.0f, 1 - 2 * t);
+    cp.sc16[2] = (float)(1.25 * 2.0 * 1.01 * ldexp(1.0, -25 - t) * sqrt((double)d));
+    cp.sc16[3] = 0.0f;
+}
+
+// fp16 tile images of 2^t b (the same b = fl(c - mu) as the bf16 images, and
+// a copy of their constants); one wavefront per (padded) center
+__global__ __launch_bounds__(256) void k_center_pack16(const float *__restrict__ C, int k, int d, int d_pad,
+                                                       int k_pad, CenterPack cp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + wave;
+    if (j >= k_pad) return;
+    const int pitch = tile_pitch(d_pad);
+    const float sc = cp.sc16[0];
+    uint8_t *img = cp.img16 + (int64_t)(j >> 5) * tile_img16_bytes(d_pad);
+    _Float16 *h = reinterpret_cast<_Float16 *>(img) + (j & 31) * pitch;
+    for (int i = lane; i < pitch; i += 64) {
+        float b = 0.0f;
+        if (j < k && i < d) b = __fsub_rn(C[(int64_t)j * d + i], cp.mu[i]);
+        h[i] = (_Float16)__fmul_rn(b, sc);  // the pad columns hold zeros
+    }
+    if (lane < 3) {
+        const float *src = reinterpret_cast<const float *>(cp.img + (int64_t)(j >> 5) * tile_img_bytes(d_pad) +
+                                                           128 * pitch);
+        float *dst = reinterpret_cast<float *>(img + 64 * pitch);
+        dst[32 * lane + (j & 31)] = src[32 * lane + (j & 31)];
+    }
+}
+
 __global__ void k_zero_i64n(int64_t *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -277,8 +324,9 @@ __global__ void k_zero_i32n(int32_t *p, int64_t n) {
 constexpr int CAND_MAX = 8;
 
 struct RecheckLists {
-    int64_t *list;   // [n] listed frames
-    int64_t *count;  // [4]: list length, bits of max |b_j|, list2 length, unused
+    int64_t *listA;  // [n] frames the coarse pass leaves undecided
+    int64_t *list;   // [n] frames the fine pass leaves undecided
+    int64_t *count;  // [8]: list length, bits of max |b_j|, list2 length, listA length, unused
     float *upper;    // [n] upper bound per list position
     int32_t *ccnt;   // [slots] candidates found per list position
     int32_t *cand;   // [slots][CAND_MAX]
@@ -292,6 +340,11 @@ __device__ __forceinline__ float f32_round_up(double v) {
     if ((double)f < v) f = __uint_as_float(f >= 0.0f ? __float_as_uint(f) + 1u
                                                    : (__float_as_uint(f) == 0x80000000u ? 1u : __float_as_uint(f) - 1u));
     return f;
+}
+
+__device__ __forceinline__ void list_frame_coarse(const RecheckLists &rl, int64_t fr) {
+    const unsigned long long pos = atomicAdd((unsigned long long *)(rl.count + 3), 1ull);
+    rl.listA[pos] = fr;
 }
 
 __device__ __forceinline__ void list_frame(const RecheckLists &rl, int64_t fr, float upper) {
@@ -364,8 +417,9 @@ __device__ __forceinline__ void track2(float &m1, float &m2, float vp, float nin
 // d_j >= d~_j - E_j >= |a|^2 + m2 - E_max, and d_best <= |a|^2 + m1 + E_best
 // (both through the reference loop's relative rounding g).  A listed frame
 // carries the upper bound of its best candidate to the candidate pass.
+template <bool COARSE>
 __device__ __forceinline__ void decide_frame(int lane, int64_t fr, bool fvalid, float m1, float m2, int mt, int k,
-                                             float na, float sa, float2 CE, float gref, float bmax,
+                                             float na, float sa, float2 CE, float c3, float gref, float bmax,
                                              const float *__restrict__ bn, int32_t *__restrict__ labels,
                                              const RecheckLists &rl) {
     int half = lane >> 5;
@@ -383,31 +437,53 @@ __device__ __forceinline__ void decide_frame(int lane, int64_t fr, bool fvalid, 
     const int e = (int)(__float_as_uint(m1) & 15u);
     const int imin = (mt >= 0 && m1 == m1 && m1 < 1e38f) ? mt * 32 + 8 * (e >> 2) + 4 * half + (e & 3) : 0x7fffffff;
     if (lane < 32 && fvalid && imin >= k) {
-        labels[fr] = 0;  // non-finite distances: leave it to the exact loop
-        list_frame(rl, fr, __uint_as_float(0x7fc00000u));
+        labels[fr] = 0;  // non-finite distances (or, in the coarse pass, fp16 overflow): the next pass
+        if (COARSE)
+            list_frame_coarse(rl, fr);
+        else
+            list_frame(rl, fr, __uint_as_float(0x7fc00000u));
     } else if (lane < 32 && fvalid) {
         const double a2 = (double)sa * sa;
         const double bnb = (double)bn[imin];
-        const double E1 = (double)CE.x * (a2 + bnb * bnb) + 2.0 * (double)CE.y * (double)sa * bnb;
-        const double Em = (double)CE.x * (a2 + (double)bmax * bmax) + 2.0 * (double)CE.y * (double)sa * bmax;
+        const double E1 = (double)CE.x * (a2 + bnb * bnb) + 2.0 * (double)CE.y * (double)sa * bnb +
+                          (double)c3 * ((double)sa + bnb);
+        const double Em = (double)CE.x * (a2 + (double)bmax * bmax) + 2.0 * (double)CE.y * (double)sa * bmax +
+                          (double)c3 * ((double)sa + (double)bmax);
         const double upper = ((double)na + (double)m1 + E1) * (1.0 + (double)gref);
         const double lower = ((double)na + (double)m2 - Em) * (1.0 - (double)gref);
         labels[fr] = imin;
-        if (!(lower > upper)) list_frame(rl, fr, f32_round_up(upper));
+        if (!(lower > upper)) {
+            if (COARSE)
+                list_frame_coarse(rl, fr);
+            else
+                list_frame(rl, fr, f32_round_up(upper));
+        }
     }
 }
 
-// CAND: pass 2 of the recheck (candidate mode): the frames are the listed
-// ones at list positions below rl.slots, and the epilogue records the
-// centers whose lower bound reaches the frame's upper bound instead of
-// tracking the minimum.
-template <int KSTEPS, int NW, int MINB, int FB, bool CAND = false>
+// The MFMA distance kernel in its three passes:
+//   PASS_COARSE: every frame; one fp16 product per K-step of 2^t a and 2^t b
+//     (fp16 images); decides with the coarse bound, lists the rest in listA.
+//   PASS_FINE: the frames of listA; bf16 hi/lo of a and b, three products
+//     per K-step; decides with the fine bound, lists the rest with their
+//     upper bound.
+//   PASS_CAND: the frames listed by PASS_FINE (positions below rl.slots);
+//     the epilogue records the centers whose lower bound reaches the frame's
+//     upper bound instead of tracking the minimum.
+//   PASS_FULL: PASS_FINE over every frame (pass 1 without a coarse pass).
+constexpr int PASS_COARSE = 0, PASS_FINE = 1, PASS_CAND = 2, PASS_FULL = 3;
+template <int KSTEPS, int NW, int MINB, int FB, int PASS>
 __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__restrict__ X, int64_t n, int d,
                                                             CenterPack cp, int k, int k_pad, float2 CE,
                                                             float gref, int32_t *__restrict__ labels,
                                                             RecheckLists rl, const int32_t *__restrict__ done) {
+    constexpr bool COARSE = PASS == PASS_COARSE, CAND = PASS == PASS_CAND;
     if (done && *done) return;
-    int64_t nf = n;  // frames (pass 1) or listed frames with a candidate slot (pass 2)
+    int64_t nf = n;  // frames: all (coarse), listA (fine), the candidate slots (cand)
+    if constexpr (PASS == PASS_FINE) {
+        nf = rl.count[3];
+        if ((int64_t)blockIdx.x * NW * FB * 32 >= nf) return;
+    }
     if constexpr (CAND) {
         const int64_t c = *rl.count;
         nf = c < rl.slots ? c : rl.slots;
@@ -415,8 +491,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     }
     constexpr int DP = KSTEPS * 16;
     constexpr int PITCH = tile_pitch(DP);
-    constexpr int IMG = tile_img_bytes(DP);
-    constexpr int MAT = 128 * PITCH;        // bytes of hi + lo in an image
+    constexpr int IMG = COARSE ? tile_img16_bytes(DP) : tile_img_bytes(DP);
+    constexpr int MAT = (COARSE ? 64 : 128) * PITCH;  // bytes of the center fragments in an image
     constexpr int MCH = MAT / 16;           // 16-byte chunks of hi + lo
     constexpr int CCH = 3 * 32 * 4 / 16;    // 16-byte chunks of nb, q, r
     constexpr int NT = NW * 64;
@@ -445,6 +521,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         fvalid[b] = fpos[b] < nf;
         if constexpr (CAND)
             frame[b] = fvalid[b] ? rl.list[fpos[b]] : 0;
+        else if constexpr (PASS == PASS_FINE)
+            frame[b] = fvalid[b] ? rl.listA[fpos[b]] : 0;
         else
             frame[b] = fpos[b];
         frow[b] = (fvalid[b] ? frame[b] : 0) * (int64_t)d;  // always a valid row
@@ -455,8 +533,9 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // asynchronous copies into an LDS buffer: hi + lo of tile tm (tm < 0: none)
     // and the constants of tile tc (tc < 0: none)
     auto fetch = [&](int tm, int tc, uint8_t *dst) __attribute__((always_inline)) {
+        const uint8_t *base = COARSE ? cp.img16 : cp.img;
         if (tm >= 0) {
-            const uint8_t *src = cp.img + (int64_t)tm * IMG;
+            const uint8_t *src = base + (int64_t)tm * IMG;
 #pragma unroll
             for (int i = 0; i < ROUNDS; ++i) {
                 const int c0 = i * NT + wave * 64;  // first chunk of this wave's 1 KB block
@@ -465,13 +544,16 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             }
         }
         if (tc >= 0 && wave == 0 && lane < CCH)
-            __builtin_amdgcn_global_load_lds(cp.img + (int64_t)tc * IMG + MAT + lane * 16, (lds_ptr_t)(dst + MAT), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(base + (int64_t)tc * IMG + MAT + lane * 16, (lds_ptr_t)(dst + MAT), 16, 0, 0);
     };
     fetch(0, -1, s_img0);
 
-    // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo (branch-free) ----
+    // ---- frame fragments: a = fl(x - mu) split into bf16 hi/lo, or (coarse)
+    // 2^t a rounded to fp16 (branch-free) ----
     bf16x8 bhi[FB][KSTEPS], blo[FB][KSTEPS];
+    half8 fh[FB][KSTEPS];
     float na[FB], sa[FB];
+    const float sc16 = COARSE ? cp.sc16[0] : 1.0f;
     // a lane's 8 dims of a K-step are consecutive in the frame row: with d a
     // multiple of 4 they are two 16-byte loads, each wholly inside the row or
     // wholly in the padding (loaded from the row's last 4 dims and masked).
@@ -484,6 +566,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
             uint32_t h[8], l[8];
+            _Float16 f[8];
             float xs[8], ms[8];
             if (vec) {
 #pragma unroll
@@ -515,12 +598,19 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
                 const uint32_t keep = ok ? 0xffffffffu : 0u;
                 const float a = __uint_as_float(__float_as_uint(__fsub_rn(xs[q], ms[q])) & keep);
                 na_part = __fadd_rn(na_part, __fmul_rn(a, a));
-                split_bf16(a, h[q], l[q]);
+                if constexpr (COARSE)
+                    f[q] = (_Float16)__fmul_rn(a, sc16);
+                else
+                    split_bf16(a, h[q], l[q]);
             }
-            uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
-            uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
-            bhi[b][s] = __builtin_bit_cast(bf16x8, hv);
-            blo[b][s] = __builtin_bit_cast(bf16x8, lv);
+            if constexpr (COARSE) {
+                fh[b][s] = half8{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+            } else {
+                uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
+                uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
+                bhi[b][s] = __builtin_bit_cast(bf16x8, hv);
+                blo[b][s] = __builtin_bit_cast(bf16x8, lv);
+            }
         }
         na[b] = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
         sa[b] = sqrtf(na[b]) * (1.0f + 1e-6f);  // |a| rounded up
@@ -551,6 +641,9 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     float m1[FB], m2[FB];
     int mt[FB];
     const float ninf = opaque_neg_inf();
+    // d~ - |a|^2 = |b|^2 + pscale * (MFMA sum): -2, or -2^(1 - 2t) for the
+    // coarse pass's scaled operands
+    const float pscale = COARSE ? cp.sc16[1] : -2.0f;
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
         m1[b] = m2[b] = INFINITY;
@@ -566,6 +659,18 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
         AccFB acc;
 #pragma unroll
         for (int b = 0; b < FB; ++b) acc.v[b] = floatx16{};
+        if constexpr (COARSE) {
+            half8 ch = *reinterpret_cast<const half8 *>(ah_base);
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) {
+                half8 chn = ch;
+                if (s + 1 < KSTEPS) chn = *reinterpret_cast<const half8 *>(ah_base + (s + 1) * 16);
+#pragma unroll
+                for (int b = 0; b < FB; ++b) acc.v[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, fh[b][s], acc.v[b], 0, 0, 0);
+                ch = chn;
+            }
+            return acc;
+        }
         // center operands read one K-step ahead of their MFMAs
         bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ah_base);
         bf16x8 al = *reinterpret_cast<const bf16x8 *>(al_base);
@@ -625,7 +730,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
             for (int b = 0; b < FB; ++b)
 #pragma unroll
                 for (int h = 0; h < 4; ++h)
-                    track2(m1[b], m2[b], pack_elem(fmaf(-2.0f, acc.v[b][4 * g + h], nba[h]), 4 * g + h), ninf);
+                    track2(m1[b], m2[b], pack_elem(fmaf(pscale, acc.v[b][4 * g + h], nba[h]), 4 * g + h), ninf);
         }
 #pragma unroll
         for (int b = 0; b < FB; ++b) mt[b] = m1[b] != prev[b] ? t : mt[b];
@@ -635,11 +740,11 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // K-steps)
     constexpr int EV = (68 * FB + KSTEPS - 1) / KSTEPS;
     auto interleave = [&]() __attribute__((always_inline)) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operand reads of K-step 0
+        __builtin_amdgcn_sched_group_barrier(0x100, COARSE ? 1 : 2, 0);  // operand reads of K-step 0
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
-            if (s + 1 < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // operands of K-step s + 1
-            __builtin_amdgcn_sched_group_barrier(0x008, 3 * FB, 0);                 // 3 FB MFMAs of K-step s
+            if (s + 1 < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, COARSE ? 1 : 2, 0);  // operands of K-step s + 1
+            __builtin_amdgcn_sched_group_barrier(0x008, (COARSE ? 1 : 3) * FB, 0);   // the MFMAs of K-step s
             __builtin_amdgcn_sched_group_barrier(0x002, EV, 0);                     // epilogue VALU
         }
     };
@@ -723,10 +828,11 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // per frame block: merge the two half-waves that hold the same frame and decide
     if constexpr (CAND) return;
     const float bmax = *cp.bmax;
+    const float c3 = COARSE ? cp.sc16[2] : 0.0f;
 #pragma unroll
     for (int b = 0; b < FB; ++b)
-        decide_frame(lane, frame[b], fvalid[b], m1[b], m2[b], mt[b], k, na[b], sa[b], CE, gref, bmax, cp.bn, labels,
-                     rl);
+        decide_frame<COARSE>(lane, frame[b], fvalid[b], m1[b], m2[b], mt[b], k, na[b], sa[b], CE, c3, gref, bmax,
+                             cp.bn, labels, rl);
 }
 
 // Resident-center form for small k d (every center tile image fits in LDS):
@@ -858,7 +964,7 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
             }
             mt = m1 != prev ? t : mt;
         }
-        decide_frame(lane, frame, fvalid, m1, m2, mt, k, na, sa, CE, gref, bmax, cp.bn, labels, rl);
+        decide_frame<false>(lane, frame, fvalid, m1, m2, mt, k, na, sa, CE, 0.0f, gref, bmax, cp.bn, labels, rl);
     }
 }
 
@@ -885,6 +991,22 @@ static float2 assign_error_coeffs(int d, int d_pad) {
     float2 ce;
     ce.x = (float)(1.25 * (c0 + gam(d)));
     ce.y = (float)(1.25 * (c0 + 3.1 * ldexp(1.0, -16) + 1.02 * gam(3.0 * d_pad)));
+    return ce;
+}
+
+// The coarse pass: one fp16 product of ah = fp16(2^t a) and bh = fp16(2^t b)
+// per dimension.  Normal fp16 values carry a relative rounding <= 2^-11
+// (|ah bh - 2^2t a_i b_i| <= (2^-10 + 2^-22) 2^2t |a_i b_i|), the FP32 MFMA
+// accumulation of d_pad exact products adds 1.01 gamma_{d_pad}; the norms and
+// the other terms are those of the fine bound (c1 unchanged).  Underflow to
+// fp16 subnormals is the absolute term c3 (|a| + |b_j|) of k_center_scale16.
+static float2 assign_error_coeffs_coarse(int d, int d_pad) {
+    const double u = ldexp(1.0, -24);
+    auto gam = [&](double m) { return m * u / (1.0 - m * u); };
+    const double c0 = 4.2 * u + 32.0 * u;
+    float2 ce;
+    ce.x = (float)(1.25 * (c0 + gam(d)));
+    ce.y = (float)(1.25 * (c0 + ldexp(1.0, -10) + ldexp(1.0, -22) + 1.01 * gam(d_pad)));
     return ce;
 }
 
@@ -917,8 +1039,8 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
     AssignWs w{};
     Carver cv(ws);
     const size_t nl = n > 0 ? (size_t)n : 1;
-    // list length, bits of max |b_j|, list2 length (zeroed per call)
-    w.rl.count = cv.take<int64_t>(4);
+    // list length, bits of max |b_j|, list2 length, listA length (zeroed per call)
+    w.rl.count = cv.take<int64_t>(8);
     w.rl.list = cv.take<int64_t>(nl);
     w.cp.bmax = w.rl.count ? reinterpret_cast<float *>(w.rl.count + 1) : nullptr;
     if (mode == EMMA_ASSIGN_BF16X3_RECHECK) {
@@ -926,13 +1048,18 @@ static AssignWs carve_assign(void *ws, int64_t n, int d, int k, int mode) {
         w.rl.upper = cv.take<float>(nl);
         w.rl.ccnt = cv.take<int32_t>((size_t)w.rl.slots);
         w.rl.cand = cv.take<int32_t>((size_t)w.rl.slots * CAND_MAX);
-        w.rl.list2 = cv.take<int64_t>(nl);
+        // listA is consumed by the fine pass before the candidate recheck
+        // writes list2: one buffer serves both
+        w.rl.listA = cv.take<int64_t>(nl);
+        w.rl.list2 = w.rl.listA;
         const int ks = pick_ksteps(d);
         const int d_pad = (ks > 0 ? ks : 1) * 16;
         const int k_pad = (int)cdiv(k, 32) * 32;
         w.cp.img = cv.take<uint8_t>((size_t)(k_pad / 32) * tile_img_bytes(d_pad));
+        w.cp.img16 = cv.take<uint8_t>((size_t)(k_pad / 32) * tile_img16_bytes(d_pad));
         w.cp.bn = cv.take<float>(k_pad);
         w.cp.mu = cv.take<float>(d_pad);
+        w.cp.sc16 = cv.take<float>(4);
     }
     w.bytes = cv.bytes();
     return w;
@@ -1019,12 +1146,19 @@ static const Variant kVarK8[] = {{4, 2, 1}, {8, 1, 1}, {4, 1, 2}};              
 static const Variant kVarMid[] = {{4, 2, 1}, {8, 1, 1}};                                              // 12 .. 19
 static const Variant kVarLarge[] = {{4, 1, 1}};                                                       // 24, 32
 
-static int n_variants(int ks) {
+static int n_shapes(int ks) {
     if (ks <= 4) return (int)(sizeof(kVarSmall) / sizeof(Variant));
     if (ks <= 8) return (int)(sizeof(kVarK8) / sizeof(Variant));
     if (ks <= 19) return (int)(sizeof(kVarMid) / sizeof(Variant));
     return (int)(sizeof(kVarLarge) / sizeof(Variant));
 }
+// variants 0 .. n_shapes - 1: the coarse pass in tile shape v (variant 0 of a
+// small K-step count: the resident fine kernel where its images fit);
+// n_shapes .. 2 n_shapes - 1: the fine pass over every frame in tile shape
+// v - n_shapes.  Which pipeline is faster depends on d, k and the data (how
+// many frames the coarse bound leaves to the fine pass), so the autotuner
+// times both.
+static int n_variants(int ks) { return 2 * n_shapes(ks); }
 
 // process-wide selection table: [K-step class][ceil(log2(k_pad / 32))]
 static std::mutex g_sel_mu;
@@ -1049,12 +1183,41 @@ static int selected_variant(int ks, int k_pad) {
     return v < n_variants(ks) ? v : 0;
 }
 
-template <int KS, int NW, int MINB, int FB = 1>
+// pass 1 over every frame (PASS_COARSE or PASS_FULL), tile shape (NW, MINB, FB)
+template <int KS, int PASS, int NW, int MINB, int FB = 1>
 static void launch_v(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE, float gref,
                      int32_t *labels, const int32_t *done, hipStream_t s) {
     const int64_t blocks = cdiv(n, 32 * NW * FB);
-    k_assign_mfma<KS, NW, MINB, FB><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref, labels,
-                                                                         w.rl, done);
+    k_assign_mfma<KS, NW, MINB, FB, PASS><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref,
+                                                                             labels, w.rl, done);
+}
+
+template <int KS, int PASS>
+static void launch_shape(int v, const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
+                         float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
+    if constexpr (KS <= 4) {
+        switch (v) {
+            case 1: launch_v<KS, PASS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 2: launch_v<KS, PASS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 3: launch_v<KS, PASS, 4, 2, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 4: launch_v<KS, PASS, 8, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 5: launch_v<KS, PASS, 4, 1, 4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            default: launch_v<KS, PASS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+        }
+    } else if constexpr (KS <= 8) {
+        switch (v) {
+            case 1: launch_v<KS, PASS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            case 2: launch_v<KS, PASS, 4, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+            default: launch_v<KS, PASS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
+        }
+    } else if constexpr (KS <= 19) {
+        if (v == 1)
+            launch_v<KS, PASS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+        else
+            launch_v<KS, PASS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+    } else {
+        launch_v<KS, PASS, 4, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+    }
 }
 
 static int device_cus() {
@@ -1065,12 +1228,15 @@ static int device_cus() {
     return cus;
 }
 
+// pass 1: variant 0 of a small K-step count with every bf16 tile image
+// resident in LDS is the resident kernel with the fine bound (it lists to the
+// fine pass's output directly, listA stays empty); every other variant is the
+// coarse streaming pass
 template <int KS>
 static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
-                           float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
+                           float2 CEc, float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     const int v = selected_variant(KS, k_pad);
     if constexpr (KS <= 8) {
-        // variant 0 with every tile image resident in LDS when they fit
         if (v == 0 && k_pad / 32 <= res_max_tiles(KS * 16)) {
             constexpr int NW = KS <= 4 ? 8 : 4;  // 8 K-steps: 4 waves, the registers of one wave per SIMD
             int64_t blocks = cdiv(n, 32 * NW);
@@ -1081,39 +1247,21 @@ static void launch_mfma_ks(const float *X, int64_t n, int d, const AssignWs &w, 
             return;
         }
     }
-    if constexpr (KS <= 4) {
-        switch (v) {
-            case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 2: launch_v<KS, 16, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 3: launch_v<KS, 4, 2, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 4: launch_v<KS, 8, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 5: launch_v<KS, 4, 1, 4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-        }
-    } else if constexpr (KS <= 8) {
-        switch (v) {
-            case 1: launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            case 2: launch_v<KS, 4, 1, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-            default: launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); return;
-        }
-    } else if constexpr (KS <= 19) {
-        if (v == 1)
-            launch_v<KS, 8, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
-        else
-            launch_v<KS, 4, 2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
-    } else {
-        launch_v<KS, 4, 1>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
-    }
+    const int ns = n_shapes(KS);
+    if (v < ns)
+        launch_shape<KS, PASS_COARSE>(v, X, n, d, w, k, k_pad, CEc, gref, labels, done, s);
+    else
+        launch_shape<KS, PASS_FULL>(v - ns, X, n, d, w, k, k_pad, CE, gref, labels, done, s);
 }
 
-// pass 2 of the recheck: the MFMA kernel in candidate mode over the listed
-// frames (grid for every candidate slot; blocks past the list exit)
-template <int KS>
-static void launch_cand_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
+// the fine pass over listA and the candidate pass over the fine pass's list:
+// grids for the most frames the lists can hold, blocks past a list's end exit
+template <int KS, int PASS>
+static void launch_list_ks(const float *X, int64_t n, int d, const AssignWs &w, int k, int k_pad, float2 CE,
                            float gref, int32_t *labels, const int32_t *done, hipStream_t s) {
     constexpr int NW = 4, MINB = KS <= 19 ? 2 : 1;
-    const int64_t blocks = cdiv(w.rl.slots, 32 * NW);
-    k_assign_mfma<KS, NW, MINB, 1, true><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref,
+    const int64_t blocks = cdiv(PASS == PASS_CAND ? w.rl.slots : n, 32 * NW);
+    k_assign_mfma<KS, NW, MINB, 1, PASS><<<(unsigned)blocks, NW * 64, 0, s>>>(X, n, d, w.cp, k, k_pad, CE, gref,
                                                                             labels, w.rl, done);
 }
 
@@ -1137,37 +1285,64 @@ int assign_run(const float *X, int64_t n, int d, const float *C, int k, int32_t 
     const int d_pad = ks * 16;
     const int k_pad = (int)cdiv(k, 32) * 32;
     const float2 CE = assign_error_coeffs(d, d_pad);
+    const float2 CEc = assign_error_coeffs_coarse(d, d_pad);
     const float gref = (float)gamma_ref(d);
     k_zero_i64n<<<1, 64, 0, s>>>(w.rl.count, 4);
     k_zero_i32n<<<(unsigned)(cdiv(w.rl.slots, 256) < 1024 ? cdiv(w.rl.slots, 256) : 1024), 256, 0, s>>>(w.rl.ccnt,
                                                                                                    w.rl.slots);
     k_center_mean<<<(unsigned)cdiv(d_pad, 64), 256, 0, s>>>(C, k, d, d_pad, w.cp.mu);
     k_center_pack<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, CE, w.cp);
+    k_center_scale16<<<1, 1, 0, s>>>(d, w.cp);
+    k_center_pack16<<<(unsigned)cdiv(k_pad, 4), 256, 0, s>>>(C, k, d, d_pad, k_pad, w.cp);
     EMMA_TRY(check_launch("assign center pack"));
     switch (ks) {
-        case 2: launch_mfma_ks<2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 4: launch_mfma_ks<4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 8: launch_mfma_ks<8>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 12: launch_mfma_ks<12>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 16: launch_mfma_ks<16>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 19: launch_mfma_ks<19>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 24: launch_mfma_ks<24>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 2: launch_mfma_ks<2>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 4: launch_mfma_ks<4>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 8: launch_mfma_ks<8>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 12: launch_mfma_ks<12>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 16: launch_mfma_ks<16>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 19: launch_mfma_ks<19>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 24: launch_mfma_ks<24>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
+        case 32: launch_mfma_ks<32>(X, n, d, w, k, k_pad, CE, CEc, gref, labels, done, s); break;
     }
     EMMA_TRY(check_launch("assign mfma"));
-    // recheck of the listed frames: candidates by the MFMA kernel (pass 2),
-    // the reference loop over them (pass 3), the full loop over every center
-    // for the rest; the grids cover the device, the kernels read the list
-    // lengths themselves
+    // the frames the coarse pass left: the fine pass (bf16 x 3), then the
+    // recheck of those it leaves: candidates by the MFMA kernel, the
+    // reference loop over them, the full loop over every center for the rest;
+    // the grids cover the device, the kernels read the list lengths themselves
     switch (ks) {
-        case 2: launch_cand_ks<2>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 4: launch_cand_ks<4>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 8: launch_cand_ks<8>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 12: launch_cand_ks<12>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 16: launch_cand_ks<16>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 19: launch_cand_ks<19>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 24: launch_cand_ks<24>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
-        case 32: launch_cand_ks<32>(X, n, d, w, k, k_pad, CE, gref, labels, done, s); break;
+        case 2:
+            launch_list_ks<2, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<2, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 4:
+            launch_list_ks<4, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<4, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 8:
+            launch_list_ks<8, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<8, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 12:
+            launch_list_ks<12, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<12, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 16:
+            launch_list_ks<16, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<16, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 19:
+            launch_list_ks<19, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<19, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 24:
+            launch_list_ks<24, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<24, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
+        case 32:
+            launch_list_ks<32, PASS_FINE>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            launch_list_ks<32, PASS_CAND>(X, n, d, w, k, k_pad, CE, gref, labels, done, s);
+            break;
     }
     EMMA_TRY(check_launch("assign candidates"));
     const int64_t eb = cdiv(n * CAND_MAX, 256) < 4096 ? cdiv(n * CAND_MAX, 256) : 4096;

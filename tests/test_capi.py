@@ -107,12 +107,13 @@ def test_api_fails_loudly_without_gpu():
 
 
 def test_assign_variant_table():
-    """Tile-variant queries and selection are host-only calls."""
+    """Variant queries and selection are host-only calls: two first-pass
+    pipelines (coarse fp16 + fine, or fine only) times the tile shapes."""
     L = _native.lib()
-    assert L.emma_assign_variants(32) == 6
-    assert L.emma_assign_variants(128) == 3
-    assert L.emma_assign_variants(300) == 2
-    assert L.emma_assign_variants(512) == 1
+    assert L.emma_assign_variants(32) == 12
+    assert L.emma_assign_variants(128) == 6
+    assert L.emma_assign_variants(300) == 4
+    assert L.emma_assign_variants(512) == 2
     assert L.emma_assign_variants(513) == 0
     assert L.emma_assign_select(300, 1000, 5) == -1
     assert L.emma_assign_select(600, 1000, 0) == -1

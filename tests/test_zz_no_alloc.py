@@ -48,10 +48,10 @@ def _data(n, d, k, seed):
 @pytest.mark.parametrize("mode", [_native.EMMA_ASSIGN_BF16X3_RECHECK, _native.EMMA_ASSIGN_FP32_PLAIN])
 @pytest.mark.parametrize("d,k,n1,n2", [(30, 100, 50_000, 20_000),       # one chunk each
                                        (300, 1000, 70_000, 9_000),      # one chunk each
-                                       (512, 12_000, 250_000, 140_000)])  # chunks of ~130k frames
+                                       (512, 8_000, 250_000, 140_000)])  # chunks of ~130k frames
 def test_noalloc_matches_workspace_entry(mode, d, k, n1, n2):
     if mode == _native.EMMA_ASSIGN_FP32_PLAIN and k > 1000:
-        pytest.skip("the plain loop at k = 12,000 takes long and has no scratch")
+        pytest.skip("the plain loop at k = 8,000 takes long and has no scratch")
     X, C = _data(n1, d, k, seed=d + k)
     for n in (n1, n2):  # second call smaller: the scratch holds the first call's contents
         Xs = X[:n].contiguous()
