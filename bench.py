@@ -73,10 +73,28 @@ def mom_executed_flops(d, P):
     return nb * (nb + 1) // 2 * 64 * 64 * 2 * 4 * 2 * float(P)
 
 
-def asg_executed_flops(d, k, n):
-    """bf16 MFMA flops of the split assignment: three products (hh, hl, lh)
-    over d padded to 16 and k padded to 32."""
-    return 3 * 2.0 * n * (-(-d // 16) * 16) * (-(-k // 32) * 32)
+def asg_first_pass_products(d, variant):
+    """MFMA products per dimension of the assignment's first pass for a tile
+    variant: 1 for the coarse fp16 pass (variants below the shape count, except
+    the resident fine kernel, variant 0 for d <= 128), 3 for the bf16 split."""
+    from pyemma_amd import _native
+
+    shapes = _native.lib().emma_assign_variants(d) // 2
+    if variant is None or variant >= shapes or (variant == 0 and d <= 128):
+        return 3
+    return 1
+
+
+def asg_executed_flops(d, k, n, products=3):
+    """MFMA flops of the assignment's first pass over all n frames: `products`
+    products (1: coarse fp16; 3: bf16 hh, hl, lh) over d padded to the K-step
+    width and k padded to 32.  The passes over the listed frames come on top."""
+    ks = -(-d // 16)
+    for s in (2, 4, 8, 12, 16, 19, 24, 32):
+        if ks <= s:
+            ks = s
+            break
+    return products * 2.0 * n * (ks * 16) * (-(-k // 32) * 32)
 
 
 def parse():
@@ -248,7 +266,7 @@ def main():
         cpu, par = cpu_baseline(args, X, offsets, C, labels, mats)
         parity.update(par)
     kern["moments"]["executed_tflop"] = mom_executed_flops(d, P) / 1e12
-    kern["assign"]["executed_tflop"] = asg_executed_flops(d, args.k, N) / 1e12
+    kern["assign"]["executed_tflop"] = asg_executed_flops(d, args.k, N, asg_first_pass_products(d, best)) / 1e12
     ex = kern[dom]["executed_tflop"] * 1e12 / (kern[dom]["ms"] * 1e-3) / 1e12
     roofline.update({"executed_tflops": ex, "executed_frac": ex / PEAK_BF16_TFLOPS})
 
@@ -501,7 +519,7 @@ def large_k_leg(args, n=10_000_000, d=512, k=4096, n_seed=200_000, iters=5, n_pl
     ms_ll = _timed(run_lloyd, reps=1)
     n_iter = int(res["out"][2].item())
     f_asg = 2.0 * d * k * n
-    ex_asg = asg_executed_flops(d, k, n)
+    ex_asg = asg_executed_flops(d, k, n, asg_first_pass_products(d, variant))
     ach = f_asg / (ms_asg * 1e-3) / 1e12
     pp_bytes = (4 * d + 8) * n_seed * (k - 1)
     out = {"workload": "%d x %d f32 Gaussian mixture, k=%d" % (n, d, k),
@@ -555,7 +573,7 @@ def batched_leg(args, ntraj=4096, d=128, lag=5, k=512, n_seed=1_000_000, n_plain
     import torch
 
     from oracle import eig as oeig
-    from pyemma_amd import _native
+    from pyemma_amd import _native, autotune
     from pyemma_amd.batch import BatchedDiscretizer
     from pyemma_amd.synthetic import langevin_trajectories
 
@@ -601,7 +619,9 @@ def batched_leg(args, ntraj=4096, d=128, lag=5, k=512, n_seed=1_000_000, n_plain
            "assign": {"ms": ms_asg, "bytes": asg_bytes, "gbs": asg_bytes / (ms_asg * 1e-3) / 1e9,
                       "hbm_frac": asg_bytes / (ms_asg * 1e-3) / 1e9 / PEAK_HBM_GBS,
                       "rechecked_frames": int(nre.item()),
-                      "executed_frac": asg_executed_flops(d, k, N) / (ms_asg * 1e-3) / 1e12 / PEAK_BF16_TFLOPS}}
+                      "executed_frac": asg_executed_flops(d, k, N, asg_first_pass_products(
+                          d, autotune.tuned().get(autotune._bucket(d, k), (None,))[0])) / (ms_asg * 1e-3) / 1e12
+                                       / PEAK_BF16_TFLOPS}}
     # parity
     lab_plain = torch.empty(n_plain, dtype=torch.int32, device=dev)
     _native.assign(X[:n_plain], C, mode=_native.EMMA_ASSIGN_FP32_PLAIN, labels=lab_plain)

@@ -152,3 +152,42 @@ def test_recheck_candidate_paths(reps):
     np.testing.assert_array_equal(lab, ref)
     assert nre == 3000
     assert np.all(ref % reps == 0)
+
+
+def _every_variant(X, C):
+    """Labels of X under every variant (both first-pass pipelines, every tile shape)."""
+    L = _native.lib()
+    d, k = X.shape[1], C.shape[0]
+    out = []
+    try:
+        for v in range(L.emma_assign_variants(d)):
+            _native.check(L.emma_assign_select(d, k, v), "emma_assign_select")
+            out.append(_run(X, C, _native.EMMA_ASSIGN_BF16X3_RECHECK)[0])
+    finally:
+        _native.check(L.emma_assign_select(d, k, -1), "emma_assign_select")
+    return out
+
+
+@pytest.mark.parametrize("d", [64, 300])
+@pytest.mark.parametrize("case", ["outliers", "tiny", "huge", "mixed_scale"])
+def test_coarse_pass_edge_cases(d, case):
+    """The coarse pass scales frames and centers by 2^t (max |b| near 2^9) into
+    fp16.  Outlier frames beyond 64 max|b| overflow fp16 and must reach the fine
+    pass; tiny (1e-30) and huge (1e30) data clamp the exponent t to +-60; frames
+    with some dimensions 1e-6 of the others underflow fp16 there.  Every
+    variant gives the oracle's labels."""
+    rng = np.random.default_rng({"outliers": 1, "tiny": 2, "huge": 3, "mixed_scale": 4}[case] + d)
+    X, C = _data(3000, d, 40, seed=d + 1)
+    if case == "outliers":
+        X[::7] *= 5000.0
+    elif case == "tiny":
+        X, C = X * np.float32(1e-30), C * np.float32(1e-30)
+    elif case == "huge":
+        X, C = X * np.float32(1e30), C * np.float32(1e30)
+    else:
+        scale = np.where(rng.random(d) < 0.3, 1e-6, 1.0).astype(np.float32)
+        X, C = X * scale, C * scale
+    X, C = np.ascontiguousarray(X, np.float32), np.ascontiguousarray(C, np.float32)
+    ref = on.assign(X, C, nthreads=4)
+    for v, lab in enumerate(_every_variant(X, C)):
+        np.testing.assert_array_equal(lab, ref, err_msg="variant %d" % v)
