@@ -111,6 +111,9 @@ def parse():
     ap.add_argument("--no-hbm-legs", action="store_true", help="skip the small-d assignment legs")
     ap.add_argument("--no-large-k", action="store_true", help="skip the 10^7 x 512, k = 4096 k-means leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-trajectory batched leg")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="only the timed steps: skip the other moments modes and the adjacent, projected and "
+                         "k-means legs (for profiler runs)")
     ap.add_argument("--cpu-frames", type=int, default=100000, help="frames in the CPU moments sample")
     ap.add_argument("--cpu-assign-frames", type=int, default=10000, help="frames in the CPU assign sample")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving leg (0: skip)")
@@ -221,8 +224,9 @@ def main():
     # and the bf16 split with single-level FP32 accumulation; their matrices
     # are kept for the eigenvalue comparison against the FP64 oracle
     mats = {"bf16x4_acc": (last["C00"], last["C0t"])}
-    for key, name, prec in (("moments_fp64", "fp64", _native.EMMA_PREC_FP64),
-                            ("moments_bf16x4", "bf16x4", _native.EMMA_PREC_BF16X4)):
+    other_modes = () if args.no_legs else (("moments_fp64", "fp64", _native.EMMA_PREC_FP64),
+                                           ("moments_bf16x4", "bf16x4", _native.EMMA_PREC_BF16X4))
+    for key, name, prec in other_modes:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         w, _, _, Mxx, Mxy, _ = _native.moments_lagged(X, off_dev, args.lag, reversible=True, remove_mean=True,
@@ -235,10 +239,11 @@ def main():
         kern[key] = {"ms": ms, "algorithmic_tflop": f_mom / 1e12, "tflops": f_mom / (ms * 1e-3) / 1e12}
         log("%s %.2f ms" % (key, ms))
 
-    kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
+    if not args.no_legs:
+        kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
+        kern["tica_projected"] = projected_leg(args, X, off_dev, mom_ms)
+        kern["kmeans"] = kmeans_leg(args, X, sub, cpu=not args.no_cpu)
     del xyz
-    kern["tica_projected"] = projected_leg(args, X, off_dev, mom_ms)
-    kern["kmeans"] = kmeans_leg(args, X, sub, cpu=not args.no_cpu)
 
     plain = None
     labels_plain = None
