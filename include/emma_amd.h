@@ -54,7 +54,7 @@ extern "C" {
 #define EMMA_PREC_BF16X4 1  /* operands split into bf16 hi+lo, 4 bf16 MFMA products, FP32 tile accumulation */
 #define EMMA_PREC_FP64 2    /* FP64 operands on v_mfma_f64_16x16x4_f64, FP64 accumulation */
 #define EMMA_PREC_BF16X4_ACC 3 /* bf16 hi+lo, 4 products, FP32 accumulation restarted every 64 pairs and summed in FP32
-                                  over 1024 pairs (every kind of moments) */
+                                  over 4096 pairs (every kind of moments) */
 
 /* assignment modes */
 #define EMMA_ASSIGN_FP32_PLAIN 0    /* sequential FP32 loop per (frame, center) on the vector ALUs */

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <type_traits>
 
 #include "../../include/emma_amd.h"
 
@@ -85,6 +86,16 @@ __device__ __forceinline__ void split_bf16x2(float a0, float a1, uint32_t &hi, u
     const f32x2_t hf = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
     const f32x2_t r = v - hf;
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
+}
+
+// f(std::integral_constant<int, K>()) for K = B .. E-1, in order: loop
+// bodies whose register-array indices must be compile-time constants.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>());
+        static_for<B + 1, E>(f);
+    }
 }
 
 // 64-lane butterfly sum (every lane ends with the same value).

@@ -20,7 +20,7 @@
 //   BF16X4  k_rev_bf16 (every kind of moments): operands split into bf16 hi +
 //           lo, products hh, hl, lh, ll on v_mfma_f32_32x32x16_bf16, FP32 tile
 //           accumulation (two-level for BF16X4_ACC), TB = 128, 8 waves, two
-//           LDS buffers, frame loads two stages ahead
+//           LDS buffers, frame loads four stages ahead
 //   FP32    k_main: v_mfma_f32_32x32x2_f32, FP32 tile accumulation, TB = 128
 //   FP64    k_main: v_mfma_f64_16x16x4_f64 on FP64 operands (x - K exact in
 //           FP64), FP64 accumulation, TB = 64
@@ -507,7 +507,11 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 // V = x - y, split both into bf16 hi + lo and write them to one of two LDS
 // buffers while the waves run the MFMAs of the previous stage from the other
 // buffer; one barrier per stage.  The raw frame values of a stage are loaded
-// two stages ahead into one of two register sets.  Wave roles:
+// RSETS = 4 stages ahead into one of four register sets (two with frame
+// weights, which need more registers).  Inside a stage the
+// 16 MFMAs of a wave are issued between the pieces of the split arithmetic
+// (scheduling barriers fix the order), so that the vector ALU works while the
+// matrix core runs.  Wave roles:
 //   off-diagonal task (I < J): wave w computes matrix w >> 2 (UU | VV) on the
 //     64 x 64 block ((w >> 1) & 1, w & 1); loader thread t owns column t & 127
 //     of block I (group (t >> 7) even) or J (odd), pairs 8 ((t >> 7) >> 1) ..
@@ -522,9 +526,11 @@ constexpr int RNT = 512;             // threads per workgroup
 constexpr int RTB = 128;             // tile edge
 constexpr int RBT = 16;              // pairs per stage
 constexpr int RPITCH = RBT + 8;      // uint16 per column row (48 B: conflict-free 16-byte reads)
-constexpr int RFLUSH = 1024;         // pairs per FP32 accumulation span
-constexpr int RSPAN = RFLUSH / RBT;  // stages per span (even)
-constexpr int RSUB = 4;              // stages per FP32 sub-span (even, divides RSPAN)
+constexpr int RSUB = 4;              // stages per loop iteration = per FP32 sub-span (two-level)
+// pairs per FP32 accumulation span: the two-level mode adds one sub-span sum
+// per RSUB stages to its span sum, the single-level mode every product
+constexpr int RFLUSH_TL = 4096;
+constexpr int RFLUSH_SL = 1024;
 
 struct RevLds {
     uint16_t op[2][4][2][RTB * RPITCH];  // [buffer][U_I, U_J, V_I, V_J][hi, lo][column row]
@@ -657,12 +663,15 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     auto stage_fast = [&](int64_t q0) __attribute__((always_inline)) -> bool {
         return q0 >= cur.p0 && q0 + PPT <= cur.p1 && q0 + PPT <= p_hi;
     };
-    float xa[PPT], ya[PPT], xb[PPT], yb[PPT];
+    // register sets of raw frame values: stage t is loaded into set t % RSETS
+    // (loads RSETS stages ahead; at most 63 vector loads may be in flight)
+    constexpr int RSETS = WT ? 2 : 4;
+    float xs[RSETS][PPT], ys[RSETS][PPT];
     // WT: lane l of a register set's weight holds the weight of the set's
     // pair l (l < PPT; pairs are consecutive in the pair-ordered weights), so
-    // one vector load per stage brings them with the frames; prep reads them
-    // back with readlane
-    float wa[1], wb[1];
+    // one vector load per stage brings them with the frames; the split reads
+    // them back with readlane
+    float ws[RSETS][1];
     const char *rx[PPT], *ry[PPT];  // rows of the stage being issued (wave-uniform)
     int64_t q0n = 0;                // first pair of the stage being issued
     auto stage_rows = [&](int t) __attribute__((always_inline)) {
@@ -704,52 +713,55 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     // weighted A operand of this loader thread (WT, off-diagonal modes REV / NRS:
     // block I; mode XY: every x')
     const bool wscale = WT && !(DIAG && !XY) && (XY || su == 0);
-    auto prep = [&](const float *xr, const float *yr, const float *wr, int buf) __attribute__((always_inline)) {
-        uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
-        uint32_t uwh[PPT / 2], uwl[PPT / 2], vwh[PPT / 2], vwl[PPT / 2];  // weighted copies (WT, diagonal)
-        float bu = 0.0f, bv = 0.0f;
-#pragma unroll
-        for (int q = 0; q < PPT; q += 2) {
-            float u0, u1, v0, v1;
-            if constexpr (NRS) {
-                u0 = __fsub_rn(xr[q], Kc);
-                u1 = __fsub_rn(xr[q + 1], Kc);
-                v0 = __fsub_rn(yr[q], Kc);
-                v1 = __fsub_rn(yr[q + 1], Kc);
-            } else {
-                u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
-                u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
-                v0 = __fsub_rn(xr[q], yr[q]);
-                v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
-            }
-            if constexpr (WT) {
-                const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q));
-                const float w1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q + 1));
-                if constexpr (DIAG && !XY) {
-                    const float a0 = __fmul_rn(w0, u0), a1 = __fmul_rn(w1, u1);
-                    const float b0 = __fmul_rn(w0, v0), b1 = __fmul_rn(w1, v1);
-                    bu = __fadd_rn(bu, __fadd_rn(a0, a1));
-                    if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(b0, b1));
-                    split_bf16x2(a0, a1, uwh[q / 2], uwl[q / 2]);
-                    split_bf16x2(b0, b1, vwh[q / 2], vwl[q / 2]);
-                } else if (wscale) {
-                    u0 = __fmul_rn(w0, u0);
-                    u1 = __fmul_rn(w1, u1);
-                    if constexpr (!XY) {
-                        v0 = __fmul_rn(w0, v0);
-                        v1 = __fmul_rn(w1, v1);
-                    }
-                }
-            } else if constexpr (DIAG) {
-                bu = __fadd_rn(bu, __fadd_rn(u0, u1));
-                if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(v0, v1));
-            }
-            split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
-            split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
+    // split of the stage being prepared: bf16 hi / lo pairs of U and V (x', y')
+    uint32_t uh[PPT / 2], ul[PPT / 2], vh[PPT / 2], vl[PPT / 2];
+    uint32_t uwh[PPT / 2], uwl[PPT / 2], vwh[PPT / 2], vwl[PPT / 2];  // weighted copies (WT, diagonal)
+    float bu = 0.0f, bv = 0.0f;  // the stage's column sums (diagonal tasks)
+    // pairs q, q + 1 of a register set (q even)
+    auto prep_pair = [&](const float *xr, const float *yr, const float *wr, int q) __attribute__((always_inline)) {
+        float u0, u1, v0, v1;
+        if constexpr (NRS) {
+            u0 = __fsub_rn(xr[q], Kc);
+            u1 = __fsub_rn(xr[q + 1], Kc);
+            v0 = __fsub_rn(yr[q], Kc);
+            v1 = __fsub_rn(yr[q + 1], Kc);
+        } else {
+            u0 = __fadd_rn(__fsub_rn(xr[q], Kc), __fsub_rn(yr[q], Kc));
+            u1 = __fadd_rn(__fsub_rn(xr[q + 1], Kc), __fsub_rn(yr[q + 1], Kc));
+            v0 = __fsub_rn(xr[q], yr[q]);
+            v1 = __fsub_rn(xr[q + 1], yr[q + 1]);
         }
+        if constexpr (WT) {
+            const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q));
+            const float w1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wr[0]), q + 1));
+            if constexpr (DIAG && !XY) {
+                const float a0 = __fmul_rn(w0, u0), a1 = __fmul_rn(w1, u1);
+                const float b0 = __fmul_rn(w0, v0), b1 = __fmul_rn(w1, v1);
+                bu = __fadd_rn(bu, __fadd_rn(a0, a1));
+                if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(b0, b1));
+                split_bf16x2(a0, a1, uwh[q / 2], uwl[q / 2]);
+                split_bf16x2(b0, b1, vwh[q / 2], vwl[q / 2]);
+            } else if (wscale) {
+                u0 = __fmul_rn(w0, u0);
+                u1 = __fmul_rn(w1, u1);
+                if constexpr (!XY) {
+                    v0 = __fmul_rn(w0, v0);
+                    v1 = __fmul_rn(w1, v1);
+                }
+            }
+        } else if constexpr (DIAG) {
+            bu = __fadd_rn(bu, __fadd_rn(u0, u1));
+            if constexpr (NRS) bv = __fadd_rn(bv, __fadd_rn(v0, v1));
+        }
+        split_bf16x2(u0, u1, uh[q / 2], ul[q / 2]);
+        split_bf16x2(v0, v1, vh[q / 2], vl[q / 2]);
+    };
+    auto prep_store = [&](int buf) __attribute__((always_inline)) {
         if constexpr (DIAG) {
             sum_u += (double)bu;
             if constexpr (NRS) sum_v += (double)bv;
+            bu = 0.0f;
+            bv = 0.0f;
         }
         const int off = lcol * RPITCH + pfirst;
         if constexpr (WT && DIAG && !XY) {  // PPT == 4: the weighted copies into slots 1 and 3
@@ -772,11 +784,12 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     };
 
     // TL (two-level FP32 accumulation): the MFMAs accumulate a sub-span of
-    // RSUB stages from zero (acc), which is then added to the span sum (sacc);
-    // the span sum goes to the FP64 slot every RFLUSH pairs.  An element of a
-    // span then carries about RSUB * 4 + RSPAN / RSUB sequential FP32
-    // roundings instead of RSPAN * 4, for 64 more registers per lane.  Without
-    // TL, acc is the span sum itself.
+    // RSUB stages from zero (acc: the first product of the sub-span takes a
+    // zero accumulator operand), which is then added to the span sum (sacc);
+    // the span sum goes to the FP64 slot every RFLUSH_TL pairs.  An element of
+    // a span then carries about RSUB * 4 + RFLUSH_TL / (RBT * RSUB) sequential
+    // FP32 roundings instead of RFLUSH_TL / RBT * 4, for 64 more registers per
+    // lane.  Without TL, acc is the span sum itself (RFLUSH_SL pairs).
     floatx16 acc[2][2], sacc[2][2];
     auto zero_acc = [&]() __attribute__((always_inline)) {
 #pragma unroll
@@ -791,16 +804,14 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
+            for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int e = 0; e < 16; ++e) sacc[r][c][e] = __fadd_rn(sacc[r][c][e], acc[r][c][e]);
-                acc[r][c] = floatx16{};
-            }
     };
-    auto mfma = [&](int buf) __attribute__((always_inline)) {
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+    auto load_ops = [&](int buf) __attribute__((always_inline)) {
         // operand reads in the order the MFMAs consume them (lo parts first),
         // so that the first products wait for two reads, not eight
-        bf16x8 ah[2], al[2], bh[2], bl[2];
         const int offr0 = (wr * 64 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
         const int offc0 = (wc * 64 + (lane & 31)) * RPITCH + (lane >> 5) * 8;
 #pragma unroll
@@ -812,37 +823,22 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         for (int r = 0; r < 2; ++r) ah[r] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sa][0][offr0 + r * 32 * RPITCH]);
 #pragma unroll
         for (int c = 0; c < 2; ++c) bh[c] = *reinterpret_cast<const bf16x8 *>(&lds.op[buf][sb][0][offc0 + c * 32 * RPITCH]);
-        // small products first
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bl[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
     };
-    // issue order inside one stage block: the operand reads, then each MFMA
-    // followed by a share of the next stage's split arithmetic and loads, so
-    // that the vector ALU work runs while the matrix core is busy
-    auto interleave = [&]() __attribute__((always_inline)) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS operand reads
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);            // 1 MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, DIAG ? 3 : 6, 0);  // VALU
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);            // 1 global load
-        }
+    // MFMA i of a stage (i = 0..15): product i >> 2 (lo lo, hi lo, lo hi, hi hi:
+    // the small products first) on the 32 x 32 block ((i >> 1) & 1, i & 1); ZC:
+    // the stage starts a sub-span, its first product starts from zero
+    auto mfma_i = [&](auto zc_tag, int i) __attribute__((always_inline)) {
+        constexpr bool ZC = decltype(zc_tag)::value;
+        const int pr = i >> 2, r = (i >> 1) & 1, c = i & 1;
+        const bf16x8 a = (pr & 1) ? ah[r] : al[r];
+        const bf16x8 b = (pr & 2) ? bh[c] : bl[c];
+        if (ZC && pr == 0)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, floatx16{}, 0, 0, 0);
+        else
+            acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[r][c], 0, 0, 0);
     };
-    auto flush = [&](bool first_flush) __attribute__((always_inline)) {
+    auto flush_as = [&](auto add_tag) __attribute__((always_inline)) {
+        constexpr bool ADD = decltype(add_tag)::value;
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -860,61 +856,119 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
                     double *dst = reinterpret_cast<double *>(base + e * 512 + loff);
                     // private slot, one writer lane per element: no atomicity needed;
                     // the L2-side add keeps the old value out of registers
-                    if (first_flush)
-                        *dst = v;
-                    else
+                    if constexpr (ADD)
                         unsafeAtomicAdd(dst, v);
+                    else
+                        *dst = v;
                 }
             }
+        // drain the slot writes here: the loop's frame loads then are the only
+        // vector memory operations in flight, and each stage waits only for
+        // the loads of its own register set
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    };
+    auto flush = [&](bool first_flush) __attribute__((always_inline)) {
+        if (first_flush)
+            flush_as(std::integral_constant<bool, false>());
+        else
+            flush_as(std::integral_constant<bool, true>());
     };
 
-    // ---- pipeline: stage s runs its MFMAs from buffer s & 1 while stage s + 1
-    // is prepared into the other buffer from register set (s + 1) & 1, which is
-    // then refilled with stage s + 3 ----
-    const int n_even = n_stages > 0 ? (n_stages + 1) & ~1 : 2;  // at least one (zero) span: every slot is written
+    // ---- pipeline: stage t runs its MFMAs from LDS buffer t & 1 while stage
+    // t + 1 is split from register set (t + 1) % RSETS into the other buffer;
+    // that set is then refilled with stage t + 1 + RSETS ----
+    // The loop runs whole iterations of RU stages (a multiple of RSETS and of
+    // RSUB, so that every set and buffer index is a constant) and at least one
+    // (zero) span, so that every slot is written; stages past the slice load
+    // the shift vector for x and y (U = V = 0).
+    constexpr int RU = RSETS * RSUB / (RSETS % RSUB == 0 ? RSUB : (RSETS % 2 == 0 ? 2 : 1));
+    static_assert(RU % RSETS == 0 && RU % RSUB == 0, "loop unroll");
+    const int rspan = (TL ? RFLUSH_TL : RFLUSH_SL) / RBT;  // stages per span, a multiple of RSUB
+    const int n_run = n_stages > 0 ? (n_stages + RU - 1) / RU * RU : RU;
     auto issue = [&](float *xr, float *yr, float *wr, int t) __attribute__((always_inline)) {
         stage_rows(t);
         issue_rows(xr, yr, wr);
     };
-    issue(xa, ya, wa, 0);
-    issue(xb, yb, wb, 1);
-    prep(xa, ya, wa, 0);
-    issue(xa, ya, wa, 2);
-    zero_acc();
-    __syncthreads();
+    // prologue (inside each of the two loop versions below, so that no load is
+    // moved across the branch between them): the loads in stage order
+    // (scheduling barriers), stage 0 split into buffer 0, then every load
+    // drained once, so that the loop's first stage waits only for its own
+    // register set (a register the prologue loads into may serve as an
+    // accumulator inside the loop)
+    auto prologue = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < RSETS; ++k) {
+            issue(xs[k], ys[k], ws[k], k);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q += 2) prep_pair(xs[0], ys[0], ws[0], q);
+        prep_store(0);
+        __builtin_amdgcn_sched_barrier(0);
+        issue(xs[0], ys[0], ws[0], RSETS);
+        __builtin_amdgcn_sched_barrier(0);
+        zero_acc();
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+    };
     bool first_flush = true;
-    // one half-iteration: MFMAs of the stage in buffer mb, preparation of the
-    // register set (xr, yr) into buffer pb, then its refill with stage t.  The
-    // rows of stage t are found first (scalar code with branches), so that the
-    // matrix block is one basic block, the same for every stage: one copy of
-    // the accumulators stays live across the loop.
-    auto half = [&](auto act_tag, int mb, float *xr, float *yr, float *wr, int pb, int t)
-                    __attribute__((always_inline)) {
+    // one stage: the MFMAs of the stage in buffer mb (ACT), the split of the
+    // register set (xr, yr, wr) into buffer pb in PPT / 2 pieces, each issued
+    // after its share of the MFMAs, then the set's refill with stage t.  The
+    // rows of stage t are found first (scalar code with branches), so that
+    // the matrix block is one basic block, the same for every stage.
+    auto stage = [&](auto act_tag, auto zc_tag, int mb, float *xr, float *yr, float *wr, int pb, int t)
+                     __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
+        constexpr int NP = PPT / 2;        // split pieces
+        constexpr int MPP = 16 / NP;       // MFMAs per piece
+        constexpr int VPM = DIAG ? 3 : 5;  // vector instructions after each MFMA
         stage_rows(t);
-        if constexpr (ACT) mfma(mb);
-        prep(xr, yr, wr, pb);
-        issue_rows(xr, yr, wr);
-        if constexpr (ACT) interleave();
+        if constexpr (ACT) load_ops(mb);
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            if constexpr (ACT) {
+#pragma unroll
+                for (int i = j * MPP; i < (j + 1) * MPP; ++i) mfma_i(zc_tag, i);
+            }
+            prep_pair(xr, yr, wr, 2 * j);
+            if (j == NP - 1) {
+                prep_store(pb);
+                issue_rows(xr, yr, wr);
+            }
+            if constexpr (ACT) {
+#pragma unroll
+                for (int i = 0; i < MPP; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // 1 MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);  // vector ALU
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
         __syncthreads();
     };
     // waves without matrix work run the same loads, splits and barriers
     auto run = [&](auto act_tag) __attribute__((always_inline)) {
         constexpr bool ACT = decltype(act_tag)::value;
-        for (int s = 0; s < n_even; s += 2) {
-            half(act_tag, 0, xb, yb, wb, 1, s + 3);  // even stage s; prepare stage s + 1
-            half(act_tag, 1, xa, ya, wa, 0, s + 4);  // odd stage s + 1; prepare stage s + 2
-            const bool span_end = ((s + 2) % RSPAN) == 0 || s + 2 >= n_even;
-            if constexpr (ACT && TL) {
-                if (span_end || ((s + 2) % RSUB) == 0) fold();
-            }
-            if (span_end) {
-                if constexpr (ACT) {
-                    flush(first_flush);
-                    zero_acc();
+        prologue();
+        for (int s = 0; s < n_run; s += RU) {
+            static_for<0, RU>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int ns = (k + 1) % RSETS;  // set of stage s + k + 1
+                stage(act_tag, std::integral_constant<bool, TL && k % RSUB == 0>(), k & 1, xs[ns], ys[ns], ws[ns],
+                      (k + 1) & 1, s + k + 1 + RSETS);
+                if constexpr (k % RSUB == RSUB - 1) {
+                    if constexpr (ACT && TL) fold();
+                    const int done = s + k + 1;  // stages accumulated
+                    if (done % rspan == 0 || done >= n_run) {
+                        if constexpr (ACT) {
+                            flush(first_flush);
+                            zero_acc();
+                        }
+                        first_flush = false;
+                    }
                 }
-                first_flush = false;
-            }
+            });
         }
     };
     if (wave_act)
