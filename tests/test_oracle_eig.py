@@ -77,3 +77,41 @@ def test_rank_truncation_drops_constant_direction():
     C = X.T @ X / len(X)
     l, R = _solvers.eig_corr(C, C * 0.5, epsilon=1e-6)
     assert len(l) == 2 and R.shape == (3, 2)
+
+
+def _whitened(C0, Ct, epsilon):
+    """The reference's route (direct.py:225-295): whitening with C0's
+    eigendecomposition, then the symmetric eigenproblem, max element positive."""
+    s, V = np.linalg.eigh(C0)
+    keep = np.abs(s) >= epsilon
+    L = V[:, keep] / np.sqrt(s[keep])
+    l, Rt = np.linalg.eigh(L.T @ Ct @ L)
+    I = np.argsort(np.abs(l))[::-1]
+    l, R = l[I], L @ Rt[:, I]
+    for j in range(R.shape[1]):
+        R[:, j] *= np.sign(R[np.argmax(np.abs(R[:, j])), j])
+    return l, R
+
+
+@pytest.mark.parametrize("smin", [1e-2, 3e-6, 1.5e-6, 0.5e-6])
+def test_cholesky_route_equals_whitening(smin):
+    """Full-rank C0 takes the Cholesky (dsygvd) route, C0 with an eigenvalue
+    near or below the cutoff the whitening route; both give the whitening's
+    eigenvalues and eigenvectors, and the cutoff drops the same directions."""
+    rng = np.random.default_rng(7)
+    d, eps = 40, 1e-6
+    Q, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    s0 = np.concatenate([np.linspace(1.0, 2.0, d - 1), [smin]])
+    C0 = (Q * s0) @ Q.T
+    C0 = 0.5 * (C0 + C0.T)
+    A = rng.standard_normal((d, d)) * 0.1
+    Ct = 0.5 * (A + A.T) + 0.3 * C0
+    assert _solvers._no_truncation(C0, eps) == (smin > 2 * eps)
+    l, R = _solvers.eig_corr(C0, Ct, epsilon=eps, sign_maxelement=True)
+    l0, R0 = _whitened(C0, Ct, eps)
+    assert len(l) == len(l0) == (d if smin >= eps else d - 1)
+    np.testing.assert_allclose(l, l0, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(R, R0, rtol=1e-6, atol=1e-6 * np.abs(R0).max())
+    # C0-orthonormal eigenvectors of Ct r = C0 r l
+    np.testing.assert_allclose(R.T @ C0 @ R, np.eye(len(l)), atol=1e-8)
+    np.testing.assert_allclose(R.T @ Ct @ R, np.diag(l), atol=1e-8)
