@@ -241,6 +241,7 @@ def main():
 
     if not args.no_legs:
         kern.update(adjacent_legs(args, X, off_dev, offsets, labels, xyz))
+        kern["streamed_features"] = streamed_leg(args, xyz, offsets, C, labels)
         kern["tica_projected"] = projected_leg(args, X, off_dev, mom_ms)
         kern["kmeans"] = kmeans_leg(args, X, sub, cpu=not args.no_cpu)
     del xyz
@@ -353,6 +354,69 @@ def adjacent_legs(args, X, off_dev, offsets, labels, xyz):
     log("featurize %.2f ms, counts %.2f ms, vamp moments %.2f ms"
         % (out["featurize"]["ms"], out["counts"]["ms"], out["vamp_moments"]["ms"]))
     torch.cuda.synchronize()
+    return out
+
+
+def streamed_leg(args, xyz, offsets, C, labels_ref, chunk=1 << 20):
+    """The TICA moments and the assignment with the features streamed from the
+    particle positions (coordinates.source(..., features=...)): every chunk of
+    about 2^20 frames is featurized on the device, its moments merged with the
+    Chan update and its frames assigned, so the 12 GB feature matrix never
+    exists (peak: one chunk of features).  Timed against the same estimators
+    on the materialized features of the same featurizer (featurize all,
+    moments, assign); eigenvalues and labels compared between the two, and the
+    labels against the headline's (whose features the generator computes in
+    FP64, so near-ties can differ)."""
+    import torch
+
+    from pyemma_amd import _solvers
+    from pyemma_amd.coordinates import AssignCenters, Featurizer, LaggedCovariance, source
+    from pyemma_amd.coordinates._data import DeviceTrajectories
+
+    na = xyz.shape[1]
+    f = Featurizer(na)
+    f.add_distances(np.arange(na))
+    trajs = [xyz[int(a):int(b)] for a, b in zip(offsets[:-1], offsets[1:])]
+    src = source(trajs, features=f, chunksize=chunk)
+    Cc = C.cpu().numpy()
+    cov_kw = dict(c00=True, c0t=True, remove_data_mean=True, reversible=True, bessel=False, lag=args.lag,
+                  precision="bf16x4-acc")
+
+    def run(data):
+        cov = LaggedCovariance(**cov_kw)
+        cov.estimate(data)
+        lab = AssignCenters(Cc).assign_device(data)
+        torch.cuda.synchronize()
+        return cov, lab
+
+    def materialized():
+        dt = DeviceTrajectories(f.transform_device(xyz), offsets)
+        return run(dt)
+
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        r = fn()
+        return r, (time.perf_counter() - t0) * 1e3
+
+    (cov_m, lab_m), ms_m = timed(materialized)
+    (cov_s, lab_s), ms_s = timed(lambda: run(src))
+    l_s, _ = _solvers.eig_corr(cov_s.C00_, cov_s.C0t_, 1e-6, sign_maxelement=True)
+    l_m, _ = _solvers.eig_corr(cov_m.C00_, cov_m.C0t_, 1e-6, sign_maxelement=True)
+    n = min(len(l_s), len(l_m))
+    mism = int((lab_s.S != lab_m.S).sum().item())
+    mism_ref = int((lab_s.S.reshape(-1) != labels_ref.reshape(-1)).sum().item())
+    out = {"ms": ms_s, "materialized_ms": ms_m, "chunk_frames": chunk, "n_chunks": src.n_chunks(lag=args.lag),
+           "peak_feature_bytes": src.chunk_bytes(),
+           "materialized_feature_bytes": int(lab_m.S.numel()) * f.dimension() * 4,
+           "eig_rel_diff_vs_materialized": float(np.max(np.abs(l_s[:n] - l_m[:n]) / np.abs(l_m[:n]))),
+           "label_mismatches_vs_materialized": mism,
+           "label_mismatches_vs_headline_features": mism_ref}
+    log("streamed features: moments + assign %.1f ms in %d chunks (features on the device at a time: %.2f GB "
+        "instead of %.1f GB; materialized featurize + moments + assign %.1f ms); eig rel diff %.1e, label "
+        "mismatches %d (vs the headline's FP64-generated features: %d)"
+        % (ms_s, out["n_chunks"], out["peak_feature_bytes"] / 1e9, out["materialized_feature_bytes"] / 1e9, ms_m,
+           out["eig_rel_diff_vs_materialized"], mism, mism_ref))
     return out
 
 

@@ -16,11 +16,13 @@ from ..exceptions import InvalidDataInStreamException
 class DeviceTrajectories(object):
     """Row-concatenated float32 trajectories on the GPU plus CSR offsets."""
 
-    def __init__(self, X, offsets):
+    def __init__(self, X, offsets, offsets_dev=None):
         t = _native.torch()
         self.X = X
         self.offsets = np.asarray(offsets, dtype=np.int64)
-        self.offsets_dev = t.as_tensor(self.offsets, device=X.device)
+        # offsets_dev: the same offsets already on the device (an upload from
+        # pageable host memory waits for the stream's queued work)
+        self.offsets_dev = t.as_tensor(self.offsets, device=X.device) if offsets_dev is None else offsets_dev
         self.d = int(X.shape[1])
 
     @property

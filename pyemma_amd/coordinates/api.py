@@ -38,7 +38,9 @@ def _weights_arg(weights, data, lag, stride, skip, ncov_max, reversible=True, fo
             if for_tica and not reversible:
                 raise ValueError("Koopman re-weighting is designed for reversible processes, set reversible=True")
             koop = KoopmanEstimator(lag=lag, stride=stride, skip=skip, ncov_max=ncov_max, precision=precision)
-            koop.estimate(data)
+            # the Koopman estimator needs every frame at once: a streamed source is featurized whole
+            from .source import FeatureSource
+            koop.estimate(data.materialize() if isinstance(data, FeatureSource) else data)
             return koop.weights
         if weights == "empirical":
             return None

@@ -12,6 +12,7 @@ import numpy as np
 
 from .. import _native
 from ._data import DeviceTrajectories, to_device
+from .source import FeatureSource
 
 log = logging.getLogger("pyemma_amd.coordinates.clustering")
 
@@ -103,15 +104,43 @@ class AbstractClustering(object):
         self.n_rechecked = int(nre.item())
         return labels
 
+    def _assign_source(self, src):
+        """Labels of a FeatureSource, one device tensor per trajectory, the
+        features of one chunk at a time (rechecked frames summed over chunks
+        on the device: the chunks do not wait for the host)."""
+        t = _native.torch()
+        C = _native.as_device_f32(self.clustercenters)
+        if C.shape[1] != src.d:
+            raise ValueError("cluster centers have wrong dimension. Have dim=%i, but input has %i"
+                             % (C.shape[1], src.d))
+        total = t.zeros(1, dtype=t.int64, device=C.device)
+        nre = t.zeros(1, dtype=t.int64, device=C.device)
+
+        def lab(dt):
+            out = _native.assign(dt.X, C, mode=ASSIGN_MODES[self.assign_mode], n_rechecked=nre)
+            total.add_(nre)
+            return out
+
+        out = src.per_trajectory(lab)
+        self.n_rechecked = int(total.item())
+        return out
+
     def assign_device(self, X):
         """Labels of X kept on the device (DeviceDtrajs), the input of pyemma_amd.dtraj's counts."""
         from ..dtraj import DeviceDtrajs
 
+        if isinstance(X, FeatureSource):
+            t = _native.torch()
+            labs = self._assign_source(X)
+            off = np.concatenate([[0], np.cumsum([len(l) for l in labs])]).astype(np.int64)
+            return DeviceDtrajs(t.cat(labs).contiguous(), off)
         dt = to_device(X)
         return DeviceDtrajs(self._assign_device(dt), dt.offsets)
 
     def transform(self, X):
         """(T, 1) int32 label columns (interface.py:157-167)."""
+        if isinstance(X, FeatureSource):
+            return [l.cpu().numpy()[:, None] for l in self._assign_source(X)]
         single = not isinstance(X, (list, tuple, DeviceTrajectories))
         dt = to_device(X)
         out = [l[:, None] for l in dt.split(self._assign_device(dt))]
@@ -120,6 +149,8 @@ class AbstractClustering(object):
     def get_output(self, stride=1, skip=0, chunk=None):
         if self._data is None:
             raise RuntimeError("no data: estimate() with data first")
+        if isinstance(self._data, FeatureSource):
+            return [l.cpu().numpy()[skip::stride, None] for l in self._assign_source(self._data)]
         dt = _strided(self._data, stride, skip)
         return [l[:, None] for l in dt.split(self._assign_device(dt))]
 
@@ -194,7 +225,8 @@ class AssignCenters(AbstractClustering):
         return "[AssignCenters centers shape=%s]" % (self.clustercenters.shape,)
 
     def estimate(self, X, **kwargs):
-        dt = to_device(X)
+        # a FeatureSource is assigned chunk by chunk (its features never all exist at once)
+        dt = X if isinstance(X, FeatureSource) else to_device(X)
         if dt.d != self.clustercenters.shape[1]:
             raise ValueError("cluster centers have wrong dimension. Have dim=%i, but input has %i"
                              % (self.clustercenters.shape[1], dt.d))
@@ -291,7 +323,10 @@ class KmeansClustering(AbstractClustering):
             elif k in ("clustercenters", "max_iter", "keep_data", "tolerance", "n_clusters", "stride", "skip",
                        "init_strategy", "fixed_seed"):
                 setattr(self, k, v)
-        dt_full = to_device(X)
+        # Lloyd needs every frame in every iteration: a FeatureSource is
+        # featurized whole, as the reference copies all frames into one buffer
+        # (kmeans.py:170-200)
+        dt_full = X.materialize() if isinstance(X, FeatureSource) else to_device(X)
         dt = _strided(dt_full, self.stride, self.skip)
         n = dt.n_frames
         if not self.n_clusters:

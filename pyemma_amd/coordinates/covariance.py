@@ -146,18 +146,27 @@ class LaggedCovariance(object):
             "yy": MomentsStorage(nsave, remove_mean=rm) if self.ctt else None,
         }
 
-    def _device_moments(self, dt):
+    def _moments_launch(self, dt, skip=None):
+        """The moments call of dt; its results stay on the device (no host sync)."""
         from .koopman import frame_weights
 
-        lag = self.lag
         wts = frame_weights(self.weights, dt)
         if wts is not None and self.ctt:
             # running_moments.py:255-256
             raise ValueError("Use of weights is not implemented for compute_YY==True")
-        w, sx, sy, Mxx, Mxy, Myy = _native.moments_lagged(
-            dt.X, dt.offsets_dev, lag, stride=self.stride, skip=self.skip, reversible=self.reversible,
+        return _native.moments_lagged(
+            dt.X, dt.offsets_dev, self.lag, stride=self.stride, skip=self.skip if skip is None else skip,
+            reversible=self.reversible,
             remove_mean=self.remove_data_mean, c0t=self.c0t, ctt=self.ctt,
             precision=PRECISIONS[self.precision], weights=wts)
+
+    def _device_moments(self, dt, skip=None):
+        return self._moments_finish(self._moments_launch(dt, skip=skip))
+
+    def _moments_finish(self, dev):
+        """Host FP64 moments of a launched call, with the constant-mean shift."""
+        lag = self.lag
+        w, sx, sy, Mxx, Mxy, Myy = dev
         h = lambda t: None if t is None else t.cpu().numpy()
         w = float(w.item())
         sx, sy, Mxx, Mxy, Myy = h(sx), h(sy), h(Mxx), h(Mxy), h(Myy)
@@ -179,7 +188,51 @@ class LaggedCovariance(object):
                 Myy = corr(Myy, sy0, sy0)
         return w, sx, sy, Mxx, Mxy, Myy
 
+    def _estimate_source(self, src, partial_fit=False):
+        """Chunks of a FeatureSource: one moments call per chunk of trajectory
+        segments (the skip is applied by the segmentation, so each call reads
+        its segments with skip 0), merged by the storage's Chan update like
+        the reference's chunks (covariance.py:219-244)."""
+        import numbers
+
+        if self.weights is not None and not isinstance(self.weights, numbers.Number) \
+                and not hasattr(self.weights, "weights") and not hasattr(self.weights, "device_weights"):
+            raise ValueError("per-trajectory weight arrays are not supported with a streamed source; "
+                             "use a weights object or a number")
+        n_chunks = src.n_chunks(lag=self.lag, stride=self.stride, skip=self.skip)
+        if n_chunks == 0:
+            longest = int(max(src.trajectory_lengths()) if src.ntraj else 0)
+            if partial_fit:
+                log.warning("Could not use data passed to partial_fit(), because no single data set "
+                            "[longest=%i] is longer than lag+skip [%i]", longest, self.lag + self.skip)
+                return self
+            raise ValueError("None single dataset [longest=%i] is longer than lag+skip [%i]."
+                             % (longest, self.lag + self.skip))
+        if self._rc is None or not partial_fit:
+            self._init_storage(n_chunks)
+        # every chunk's moments call is queued before any result is read back,
+        # so the device runs the chunks back to back; the storage then merges
+        # them in chunk order
+        pending = [self._moments_launch(dt, skip=0)
+                   for _, dt in src.iter_chunks(lag=self.lag, stride=self.stride, skip=self.skip)]
+        for dev in pending:
+            self._store(*self._moments_finish(dev))
+        self._estimated = True
+        return self
+
+    def _store(self, w, sx, sy, Mxx, Mxy, Myy):
+        if self._rc["xx"] is not None:
+            self._rc["xx"].store(Moments(w, sx, sx, Mxx))
+        if self._rc["xy"] is not None:
+            self._rc["xy"].store(Moments(w, sx, sy, Mxy))
+        if self._rc["yy"] is not None:
+            self._rc["yy"].store(Moments(w, sy, sy, Myy))
+
     def _estimate(self, data, partial_fit=False):
+        from .source import FeatureSource
+
+        if isinstance(data, FeatureSource):
+            return self._estimate_source(data, partial_fit=partial_fit)
         dt = to_device(data)
         usable = [n for n in dt.lengths if n > self.lag + self.skip]
         if not usable:
@@ -192,13 +245,7 @@ class LaggedCovariance(object):
                              % (longest, self.lag + self.skip))
         if self._rc is None or not partial_fit:
             self._init_storage(len(usable))
-        w, sx, sy, Mxx, Mxy, Myy = self._device_moments(dt)
-        if self._rc["xx"] is not None:
-            self._rc["xx"].store(Moments(w, sx, sx, Mxx))
-        if self._rc["xy"] is not None:
-            self._rc["xy"].store(Moments(w, sx, sy, Mxy))
-        if self._rc["yy"] is not None:
-            self._rc["yy"].store(Moments(w, sy, sy, Myy))
+        self._store(*self._device_moments(dt))
         self._estimated = True
         return self
 

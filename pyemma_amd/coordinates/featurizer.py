@@ -151,6 +151,30 @@ class Featurizer(object):
             outs.append(Y)
         return outs[0] if len(outs) == 1 else t.cat(outs, dim=1)
 
+    def transform_device_into(self, xyz, out):
+        """Features of a contiguous (n_frames, n_atoms, 3) device tensor written
+        into out, a C-contiguous (n_frames, dim) float32 device tensor (rows of a
+        larger buffer); one feature kind is written in place, several are
+        concatenated first."""
+        if len(self.active_features) != 1:
+            out.copy_(self.transform_device(xyz))
+            return out
+        f = self.active_features[0]
+        t = _native.torch()
+        if xyz.dtype != t.float32 or xyz.dim() != 3 or xyz.shape[1] != self.n_atoms or xyz.shape[2] != 3:
+            raise ValueError("coordinates must be float32 of shape (n_frames, %d, 3)" % self.n_atoms)
+        if out.dtype != t.float32 or tuple(out.shape) != (xyz.shape[0], f.dimension) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float32 tensor of shape (%d, %d)" % (xyz.shape[0], f.dimension))
+        pairs = getattr(f, "_pairs_dev", None)
+        if pairs is None or pairs.device != xyz.device:
+            # uploaded once: an upload from pageable host memory waits for the stream
+            pairs = f._pairs_dev = t.as_tensor(f.distance_indexes.astype(np.int32), device=xyz.device).contiguous()
+        _native.check(_native.lib().emma_pair_distances(
+            _native.ptr(xyz), int(xyz.shape[0]), self.n_atoms, _native.ptr(pairs), f.dimension,
+            _native.EMMA_DIST_INVERSE if f.inverse else 0, _native.ptr(out), _native.stream_ptr()),
+            "emma_pair_distances")
+        return out
+
     def transform(self, xyz):
         """Host float32 features of one (n_frames, n_atoms, 3) array, or a list of them."""
         if isinstance(xyz, (list, tuple)):

@@ -12,6 +12,7 @@ from .._solvers import eig_corr
 from ..exceptions import ZeroRankError
 from ._data import DeviceTrajectories, to_device
 from .covariance import LaggedCovariance
+from .source import FeatureSource
 
 log = logging.getLogger("pyemma_amd.coordinates.transform.tica.TICA")
 
@@ -115,7 +116,8 @@ class TICA(object):
                 self.chunksize = v
             elif hasattr(self, k):
                 setattr(self, k, v)
-        dt = to_device(X)
+        # a FeatureSource is streamed chunk by chunk (its features never all exist at once)
+        dt = X if isinstance(X, FeatureSource) else to_device(X)
         if not self.dim <= dt.d:
             raise RuntimeError("requested more output dimensions (%i) than dimension of input data (%i)"
                                % (self.dim, dt.d))
@@ -132,7 +134,7 @@ class TICA(object):
 
     def partial_fit(self, X):
         """tica.py:165-195: eigendecomposition deferred to first property access."""
-        dt = to_device(X)
+        dt = X if isinstance(X, FeatureSource) else to_device(X)
         if not self.dim <= dt.d:
             raise RuntimeError("requested more output dimensions (%i) than dimension of input data (%i)"
                                % (self.dim, dt.d))
@@ -245,6 +247,8 @@ class TICA(object):
 
     def transform(self, X):
         """Project X (ndarray or list) onto the leading components (_tica_base.py:117-133)."""
+        if isinstance(X, FeatureSource):
+            return [y.cpu().numpy() for y in X.per_trajectory(self._project_device)]
         single = isinstance(X, np.ndarray) or (not isinstance(X, (list, tuple, DeviceTrajectories)))
         dt = to_device(X)
         Y = self._project_device(dt)
@@ -260,8 +264,11 @@ class TICA(object):
         """Projected estimation data, one array per trajectory."""
         if self._data is None:
             raise RuntimeError("no data: estimate() with data first")
-        Y = self._project_device(self._data)
-        out = self._data.split(Y)
+        if isinstance(self._data, FeatureSource):
+            out = [y.cpu().numpy() for y in self._data.per_trajectory(self._project_device)]
+        else:
+            Y = self._project_device(self._data)
+            out = self._data.split(Y)
         return [y[skip::stride][:, dimensions] for y in out]
 
     def fit_transform(self, X, **kwargs):
