@@ -924,6 +924,9 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
         constexpr int MPP = 16 / NP;       // MFMAs per piece
         constexpr int VPM = DIAG ? 3 : 5;  // vector instructions after each MFMA
         stage_rows(t);
+        // raised wave priority while the stage's MFMAs are issued (the two
+        // waves of a SIMD then tend to alternate: measured 1 % faster)
+        if constexpr (ACT) __builtin_amdgcn_s_setprio(1);
         if constexpr (ACT) load_ops(mb);
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
@@ -945,6 +948,7 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
             }
             __builtin_amdgcn_sched_barrier(0);
         }
+        if constexpr (ACT) __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     };
     // waves without matrix work run the same loads, splits and barriers
