@@ -512,13 +512,13 @@ __global__ __launch_bounds__(NTHR) void k_main(Params p, int task_base, int n_la
 // 16 MFMAs of a wave are issued between the pieces of the split arithmetic
 // (scheduling barriers fix the order), so that the vector ALU works while the
 // matrix core runs.  Wave roles:
-//   off-diagonal task (I < J): wave w computes matrix w >> 2 (UU | VV) on the
-//     64 x 64 block ((w >> 1) & 1, w & 1); loader thread t owns column t & 127
-//     of block I (group (t >> 7) even) or J (odd), pairs 8 ((t >> 7) >> 1) ..
+//   off-diagonal task (I < J): each wave computes one matrix (UU | VV) on one
+//     64 x 64 block of the tile (the blocks with matrix work on waves 0, 1, 2,
+//     ... first); loader thread t owns column t & 127 of block I (group
+//     (t >> 7) even) or J (odd), pairs 8 ((t >> 7) >> 1) ..
 //   diagonal task (I == J): only block I is staged and serves as both MFMA
-//     operands; wave w computes matrix w & 1 on block (w >> 2, (w >> 1) & 1),
-//     the block below the diagonal is skipped; loader thread t owns column
-//     t & 127, pairs 4 (t >> 7) ..
+//     operands; the blocks below the diagonal are skipped; loader thread t
+//     owns column t & 127, pairs 4 (t >> 7) ..
 // The rows of a pair come from a per-wave cursor over the trajectory table
 // (the pairs of a wave are the same for all its lanes and increase
 // monotonically), so no pair table is staged.
@@ -590,29 +590,62 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     const int64_t lagd = (int64_t)p.lag * p.d;
 
     // ---- MFMA role ----
+    // The 8 blocks (matrix m, row half wr, column half wc) of the tile, in the
+    // order b -> (b >> 2, (b >> 1) & 1, b & 1) (diagonal tiles: (b & 1,
+    // b >> 2, (b >> 1) & 1)); a block has matrix work when it lies inside d
+    // (and on or above the diagonal, and is wanted).  The blocks with matrix
+    // work go to waves 0, 1, 2, ... and the others after them, so that a tile
+    // with fewer than 8 such blocks (the last column block less than half
+    // full, or one matrix not wanted) spreads them over the four SIMDs (wave w
+    // of the workgroup runs on SIMD w % 4) instead of doubling up on two.
+    auto block_of = [&](int b, int &bm, int &bwr, int &bwc) __attribute__((always_inline)) {
+        if (DIAG && !XY) {
+            bm = b & 1;
+            bwr = b >> 2;
+            bwc = (b >> 1) & 1;
+        } else {
+            bm = b >> 2;
+            bwr = (b >> 1) & 1;
+            bwc = b & 1;
+        }
+    };
+    auto block_active = [&](int bm, int bwr, int bwc) __attribute__((always_inline)) -> bool {
+        if constexpr (XY) {
+            const int ri = bm == 0 ? I : J, ci = bm == 0 ? J : I;
+            return (ri * TB + bwr * 64 < p.d) && (ci * TB + bwc * 64 < p.d) && !(DIAG && bm == 1);
+        } else {
+            return (I * TB + bwr * 64 < p.d) && (J * TB + bwc * 64 < p.d) && !(DIAG && bwr > bwc) &&
+                   !(NRS && bm == 1 && !p.ctt);
+        }
+    };
     int m, wr, wc;
-    if (DIAG && !XY) {
-        m = wave & 1;
-        wr = wave >> 2;
-        wc = (wave >> 1) & 1;
-    } else {
-        m = wave >> 2;
-        wr = (wave >> 1) & 1;
-        wc = wave & 1;
-    }
     bool wave_act;
+    {
+        int n_act = 0;
+        for (int b = 0; b < 8; ++b) {
+            int bm, bwr, bwc;
+            block_of(b, bm, bwr, bwc);
+            n_act += block_active(bm, bwr, bwc) ? 1 : 0;
+        }
+        int ia = 0, ii = n_act, pick = 0;
+        for (int b = 0; b < 8; ++b) {
+            int bm, bwr, bwc;
+            block_of(b, bm, bwr, bwc);
+            const int pos = block_active(bm, bwr, bwc) ? ia++ : ii++;
+            if (pos == wave) pick = b;
+        }
+        block_of(pick, m, wr, wc);
+        wave_act = wave < n_act;
+    }
     int sa, sb;
     double *mslot = slot + (int64_t)m * TB * TB;  // output matrix of this wave
     if constexpr (XY) {
         const int ri = m == 0 ? I : J, ci = m == 0 ? J : I;  // row (x') and column (y') blocks
-        wave_act = (ri * TB + wr * 64 < p.d) && (ci * TB + wc * 64 < p.d) && !(DIAG && m == 1);
         sa = DIAG ? 0 : (m == 0 ? 0 : 1);  // x' of the row block
         sb = DIAG ? 2 : (m == 0 ? 3 : 2);  // y' of the column block
         const int txy = p.n_sym + ri * p.nb + ci;
         mslot = p.slots + ((int64_t)slice * p.n_tasks + txy) * p.slot_doubles;
     } else {
-        wave_act = (I * TB + wr * 64 < p.d) && (J * TB + wc * 64 < p.d) && !(DIAG && wr > wc) &&
-                   !(NRS && m == 1 && !p.ctt);
         sa = m == 0 ? 0 : 2;
         sb = DIAG ? sa : sa + 1;
         if (WT && DIAG) sa += 1;  // the weighted copy
@@ -674,11 +707,28 @@ __device__ __forceinline__ void rev_body(const Params &p, RevLds &lds, double (&
     float ws[RSETS][1];
     const char *rx[PPT], *ry[PPT];  // rows of the stage being issued (wave-uniform)
     int64_t q0n = 0;                // first pair of the stage being issued
+    // A run of common-case stages: when a stage's pairs lie in the cursor's
+    // trajectory, so do those of the next fast_left stages (whose rows follow
+    // at RBT * stride rows per stage), so their first rows come from one add
+    // instead of the pair -> row multiplications and the range checks.
+    int fast_left = 0;
+    const char *bx_run = Xb;  // x row of the next stage of the run
+    const int64_t stage_b = (int64_t)RBT * step_b;
     auto stage_rows = [&](int t) __attribute__((always_inline)) {
         const int64_t q0 = stage_q0(t);
         q0n = q0;
-        if (stage_fast(q0)) {
-            const char *bx = Xb + (cur.base + (q0 - cur.p0) * p.stride) * p.d * 4;
+        bool fast = fast_left > 0;
+        const char *bx = bx_run;
+        if (fast) {
+            --fast_left;
+        } else if (stage_fast(q0)) {
+            fast = true;
+            bx = Xb + (cur.base + (q0 - cur.p0) * p.stride) * p.d * 4;
+            const int64_t lim = cur.p1 < p_hi ? cur.p1 : p_hi;
+            fast_left = (int)((lim - q0 - PPT) / RBT);
+        }
+        if (fast) {
+            bx_run = bx + stage_b;
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 rx[q] = bx + q * step_b;
