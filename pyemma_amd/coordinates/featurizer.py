@@ -1,7 +1,7 @@
 """Distance features of particle coordinates on the GPU.
 
 The producer of the distance features the moments and assignment kernels
-consume (cfg2's 300 features are the pairwise distances of 25 particles).
+consume (300 features are the pairwise distances of 25 particles).
 Mirrors the distance part of MDFeaturizer
 (pyemma/coordinates/data/featurization/featurizer.py:198-342):
 `pairs`, `add_distances`, `add_inverse_distances`, `dimension`,
