@@ -560,8 +560,79 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // Only for small K-step counts: with many steps the compiler keeps every
     // loaded vector live at once and runs out of registers.
     const bool vec = KSTEPS <= 8 && (d & 3) == 0;
+    // The coarse pass with more K-steps (16-byte aligned rows): 16-byte loads
+    // of the frames in groups of GK K-steps, the next group's loads issued
+    // before the current group is converted, so that the frame loads of a
+    // wave are not one dword at a time (about 10 loads in flight otherwise,
+    // and the frame phase of a workgroup then takes longer than its tiles).
+    // mu of a group is loaded before the next group's frames (loads complete
+    // in order), and the values and the order of the |a|^2 terms are those of
+    // the general loop below.
+    constexpr int GK = 4;
+    constexpr int NG = (KSTEPS + GK - 1) / GK;
+    const bool gvec = COARSE && KSTEPS > 8 && (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+    if (gvec) {
+#pragma unroll
+        for (int b = 0; b < FB; ++b) {
+            float na_part = 0.0f;
+            float4 xg[2][GK][2];
+            auto load_group = [&](int g, int slot) __attribute__((always_inline)) {
+#pragma unroll
+                for (int j = 0; j < GK; ++j) {
+                    const int s = g * GK + j;
+                    if (s < KSTEPS) {
+#pragma unroll
+                        for (int hv = 0; hv < 2; ++hv) {
+                            const int i0 = s * 16 + kh + 4 * hv;
+                            xg[slot][j][hv] = *reinterpret_cast<const float4 *>(X + frow[b] + (i0 < d ? i0 : d - 4));
+                        }
+                    }
+                }
+            };
+            load_group(0, 0);
+            float4 mg[GK][2];
+            static_for<0, NG>([&](auto gc) __attribute__((always_inline)) {
+                constexpr int g = decltype(gc)::value;
+#pragma unroll
+                for (int j = 0; j < GK; ++j) {
+                    const int s = g * GK + j;
+                    if (s < KSTEPS) {
+                        mg[j][0] = *reinterpret_cast<const float4 *>(cp.mu + s * 16 + kh);
+                        mg[j][1] = *reinterpret_cast<const float4 *>(cp.mu + s * 16 + kh + 4);
+                    }
+                }
+                if constexpr (g + 1 < NG) load_group(g + 1, (g + 1) & 1);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < GK; ++j) {
+                    const int s = g * GK + j;
+                    if (s < KSTEPS) {
+                        const float4 *xv = xg[g & 1][j];
+                        const float xs[8] = {xv[0].x, xv[0].y, xv[0].z, xv[0].w, xv[1].x, xv[1].y, xv[1].z, xv[1].w};
+                        const float ms[8] = {mg[j][0].x, mg[j][0].y, mg[j][0].z, mg[j][0].w,
+                                             mg[j][1].x, mg[j][1].y, mg[j][1].z, mg[j][1].w};
+                        _Float16 f[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int i = s * 16 + kh + q;
+                            const bool ok = fvalid[b] && i < d;
+                            const uint32_t keep = ok ? 0xffffffffu : 0u;
+                            const float a = __uint_as_float(__float_as_uint(__fsub_rn(xs[q], ms[q])) & keep);
+                            na_part = __fadd_rn(na_part, __fmul_rn(a, a));
+                            f[q] = (_Float16)__fmul_rn(a, sc16);
+                        }
+                        fh[b][s] = half8{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            na[b] = __fadd_rn(na_part, __shfl_xor(na_part, 32, 64));
+            sa[b] = sqrtf(na[b]) * (1.0f + 1e-6f);  // |a| rounded up
+        }
+    }
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
+        if (gvec) break;
         float na_part = 0.0f;
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) {
