@@ -563,8 +563,9 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // The coarse pass with more K-steps (16-byte aligned rows): 16-byte loads
     // of the frames in groups of GK K-steps, the next group's loads issued
     // before the current group is converted, so that the frame loads of a
-    // wave are not one dword at a time (about 10 loads in flight otherwise,
-    // and the frame phase of a workgroup then takes longer than its tiles).
+    // wave are not one dword at a time (about 10 loads in flight otherwise:
+    // the frame phase of a workgroup is then latency-bound, and up to
+    // k = 512 it takes longer than the workgroup's tiles).
     // mu of a group is loaded before the next group's frames (loads complete
     // in order), and the values and the order of the |a|^2 terms are those of
     // the general loop below.
