@@ -560,8 +560,9 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // Only for small K-step counts: with many steps the compiler keeps every
     // loaded vector live at once and runs out of registers.
     const bool vec = KSTEPS <= 8 && (d & 3) == 0;
-    // The coarse pass with more K-steps (16-byte aligned rows): 16-byte loads
-    // of the frames in groups of GK K-steps, the next group's loads issued
+    // More K-steps (16-byte aligned rows; the bf16 passes only in tile
+    // shapes of one wave per SIMD, whose registers hold the load groups next
+    // to the hi / lo fragments): 16-byte loads of the frames in groups of GK K-steps, the next group's loads issued
     // before the current group is converted, so that the frame loads of a
     // wave are not one dword at a time (about 10 loads in flight otherwise:
     // the frame phase of a workgroup is then latency-bound, and up to
@@ -571,7 +572,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // the general loop below.
     constexpr int GK = 4;
     constexpr int NG = (KSTEPS + GK - 1) / GK;
-    const bool gvec = COARSE && KSTEPS > 8 && (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+    const bool gvec = (COARSE || NW * MINB <= 4) && KSTEPS > 8 && (d & 3) == 0 &&
+                      ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
     if (gvec) {
 #pragma unroll
         for (int b = 0; b < FB; ++b) {
@@ -613,6 +615,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
                         const float ms[8] = {mg[j][0].x, mg[j][0].y, mg[j][0].z, mg[j][0].w,
                                              mg[j][1].x, mg[j][1].y, mg[j][1].z, mg[j][1].w};
                         _Float16 f[8];
+                        uint32_t h[8], l[8];
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int i = s * 16 + kh + q;
@@ -620,9 +623,19 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
                             const uint32_t keep = ok ? 0xffffffffu : 0u;
                             const float a = __uint_as_float(__float_as_uint(__fsub_rn(xs[q], ms[q])) & keep);
                             na_part = __fadd_rn(na_part, __fmul_rn(a, a));
-                            f[q] = (_Float16)__fmul_rn(a, sc16);
+                            if constexpr (COARSE)
+                                f[q] = (_Float16)__fmul_rn(a, sc16);
+                            else
+                                split_bf16(a, h[q], l[q]);
                         }
-                        fh[b][s] = half8{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+                        if constexpr (COARSE) {
+                            fh[b][s] = half8{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+                        } else {
+                            uint4 hv = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
+                            uint4 lv = make_uint4(pack2(l[0], l[1]), pack2(l[2], l[3]), pack2(l[4], l[5]), pack2(l[6], l[7]));
+                            bhi[b][s] = __builtin_bit_cast(bf16x8, hv);
+                            blo[b][s] = __builtin_bit_cast(bf16x8, lv);
+                        }
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
