@@ -559,7 +559,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_assign_mfma(const float *__re
     // wholly in the padding (loaded from the row's last 4 dims and masked).
     // Only for small K-step counts: with many steps the compiler keeps every
     // loaded vector live at once and runs out of registers.
-    const bool vec = KSTEPS <= 8 && (d & 3) == 0;
+    const bool vec = KSTEPS <= 8 && (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
     // More K-steps (16-byte aligned rows; the bf16 passes only in tile
     // shapes of one wave per SIMD, whose registers hold the load groups next
     // to the hi / lo fragments): 16-byte loads of the frames in groups of GK K-steps, the next group's loads issued
@@ -978,7 +978,7 @@ __global__ __launch_bounds__(NW * 64, 1) void k_assign_res(const float *__restri
     const int64_t stride = (int64_t)gridDim.x * NW;
     // raw frame values of the wave's next block are loaded while the current
     // block runs its tiles (lanes: frame lane & 31, dims s * 16 + kh .. + 7)
-    const bool vec = KSTEPS <= 8 && (d & 3) == 0;
+    const bool vec = KSTEPS <= 8 && (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
     float xr[KSTEPS][8];
     auto load_raw = [&](int64_t b) __attribute__((always_inline)) {
         const int64_t fr = b * 32 + (lane & 31);

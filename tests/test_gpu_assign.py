@@ -137,6 +137,36 @@ def test_every_tile_variant_bit_exact(d, k):
         _native.check(L.emma_assign_select(d, k, 0), "emma_assign_select")
 
 
+@pytest.mark.parametrize("d", [64, 128, 300, 301, 510, 512])
+@pytest.mark.parametrize("aligned", [True, False])
+def test_frame_load_forms(d, aligned):
+    """Both frame-load forms of the MFMA passes give the oracle's labels in every tile
+    variant: 16-byte loads (d % 4 == 0 and 16-byte aligned frames; in groups of
+    K-steps beyond 8 K-steps) and the dword loads otherwise (d % 4 != 0, or frames
+    starting 4 bytes past a 16-byte boundary)."""
+    t = _native.torch()
+    L = _native.lib()
+    n, k = 3 * 1024 + 45, 160
+    X, C = _data(n, d, k, seed=d + (0 if aligned else 1))
+    ref = on.assign(X, C, nthreads=4)
+    if aligned:
+        Xt = _native.as_device_f32(X)
+    else:
+        buf = t.empty(n * d + 1, dtype=t.float32, device=_native.device())
+        Xt = buf[1:].view(n, d)
+        Xt.copy_(t.from_numpy(X))
+        assert Xt.data_ptr() % 16 == 4
+    Ct = _native.as_device_f32(C)
+    try:
+        for v in range(L.emma_assign_variants(d)):
+            _native.check(L.emma_assign_select(d, k, v), "emma_assign_select")
+            lab = _native.assign(Xt, Ct, mode=_native.EMMA_ASSIGN_BF16X3_RECHECK)
+            t.cuda.synchronize()
+            np.testing.assert_array_equal(lab.cpu().numpy(), ref, err_msg="variant %d" % v)
+    finally:
+        _native.check(L.emma_assign_select(d, k, 0), "emma_assign_select")
+
+
 @pytest.mark.parametrize("reps", [2, 8, 9, 12])
 def test_recheck_candidate_paths(reps):
     """Every center repeated `reps` times: each frame ties between `reps`
